@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the tree-likelihood hot path on one MI355X.
+
+Workload (default): 128-taxon Yule tree, 100,000 simulated DNA sites
+compressed to ~60k patterns with integer weights, GTR+Gamma4 (alpha 0.5),
+FP32 CLVs with power-of-two site scaling, compensated FP64 site sums.  A
+step is one full-traversal lnL evaluation (every CLV recomputed) with all
+data resident in HBM.  Also reported: single-branch partial re-evaluation
+throughput (re-root at a changed edge, as an MC3 branch-length move does),
+the plain one-launch-per-node path, |dlnL| against the FP64 oracle, the
+roofline of the traversal kernel and the oracle's CPU throughput.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--partial-steps", type=int, default=400)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the likelihood on a single "
+              "device (see DESIGN.md for how the path shards); --gpus %d is not "
+              "supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    from crux_amd import _native as nat
+    from crux_amd import synth
+    from crux_amd.lik import Lik
+
+    t0 = time.time()
+    prob = synth.make_problem(128, 100000, 2, mean_len=0.025)
+    S = prob.alignment.nchars
+    log("problem: 128 taxa, %d patterns, %.1fs" % (S, time.time() - t0))
+
+    ctx = nat.get_context(0)
+    lik = Lik(prob.tree, prob.alignment, ncat=4)
+    synth.configure(lik, prob.model)
+    root = prob.tree.base
+    lnl0 = lik.lnL(root)
+
+    # --- full traversals (fused) --------------------------------------------
+    def full():
+        lik.flush()
+        return lik.lnL(root)
+
+    for _ in range(args.warmup):
+        full()
+    ctx.set_timing(True)
+    full()
+    tm = ctx.last_timing()
+    traffic, nodes = ctx.last_traffic()
+    ctx.set_timing(False)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        full()
+    dt = time.perf_counter() - t
+    full_eps = args.steps / dt
+    log("full: %.1f evals/s, %.3f ms/eval; kernel ms P=%.3f trav=%.3f fin=%.3f" %
+        (full_eps, 1e3 * dt / args.steps, tm[0], tm[1], tm[2]))
+
+    # traversal kernel timing over many launches with HIP events
+    ctx.set_timing(True)
+    tr = []
+    for _ in range(20):
+        full()
+        tr.append(ctx.last_timing()[1])
+    ctx.set_timing(False)
+    trav_ms = float(np.mean(tr))
+    achieved = traffic / (trav_ms * 1e-3) / 1e9
+
+    # --- plain path: one launch per node update ---------------------------
+    ctx.set_mode(1)
+    for _ in range(3):
+        full()
+    t = time.perf_counter()
+    n_plain = max(10, args.steps // 4)
+    for _ in range(n_plain):
+        full()
+    plain_eps = n_plain / (time.perf_counter() - t)
+    ctx.set_mode(0)
+    log("plain: %.1f evals/s" % plain_eps)
+
+    # --- single-branch partial re-evaluation -----------------------------
+    rng = np.random.default_rng(7)
+    edges = list(prob.tree.edges)
+    lam = 2.0 * math.log(1.6)
+    lik.lnL(root)
+
+    def partial():
+        e = edges[int(rng.integers(len(edges)))]
+        e.length = e.length * math.exp(lam * (rng.random() - 0.5))
+        v = lik.lnL(e.ring.node)
+        prob.tree.base = e.ring.node
+        return v
+
+    for _ in range(20):
+        partial()
+    t = time.perf_counter()
+    for _ in range(args.partial_steps):
+        partial()
+    part_eps = args.partial_steps / (time.perf_counter() - t)
+    log("partial: %.1f evals/s" % part_eps)
+
+    # --- FP64 oracle: parity and CPU baseline ---------------------------------
+    cpu = None
+    dl_abs = dl_rel = None
+    if not args.no_cpu:
+        from oracle.oracle import OracleLik
+        ncpu = min(os.cpu_count() or 1, 16)
+        prob2 = synth.make_problem(128, 100000, 2, mean_len=0.025)
+        ora = OracleLik(prob2.tree, prob2.alignment, ncat=4, nthreads=ncpu)
+        synth.configure(ora, prob2.model)
+        lik2 = Lik(prob2.fresh_tree(), prob2.alignment, ncat=4)
+        synth.configure(lik2, prob2.model)
+        dev_l = lik2.lnL()
+        t = time.perf_counter()
+        ref_l = ora.lnL()
+        n = 1
+        while time.perf_counter() - t < args.cpu_seconds and n < 50:
+            ora.lnL()
+            n += 1
+        cpu_eps = n / (time.perf_counter() - t)
+        dl_abs = abs(dev_l - ref_l)
+        dl_rel = dl_abs / abs(ref_l)
+        log("oracle: lnL %.6f dev %.6f |d| %.3e rel %.3e; %.2f evals/s on %d threads" %
+            (ref_l, dev_l, dl_abs, dl_rel, cpu_eps, ncpu))
+        cpu = {"value": round(cpu_eps, 3), "unit": "lnL evals/s", "cores": ncpu,
+               "kind": "port",
+               "sample": "%d full-traversal evaluations of the same 128-taxon, %d-pattern "
+                         "GTR+G4 workload by the FP64 C restatement of the reference "
+                         "engine (oracle/cxlik_oracle.c) with %d stripe threads" % (n, S, ncpu)}
+
+    line = {
+        "metric": "lnL evals/s (full traversal, 128 taxa x ~60k patterns GTR+G4)",
+        "value": round(full_eps, 2),
+        "unit": "lnL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / full_eps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 CLV / f64 P,log-sum",
+        "data": "synthetic (Yule tree, GTR+G4 simulated sites, seeded)",
+        "config": {"workload": "single large DNA likelihood: 128 taxa, 100,000 sites -> "
+                               "%d patterns, GTR+G4 alpha=0.5, full traversal" % S,
+                   "patterns": S, "taxa": 128, "categories": 4},
+        "partial_evals_per_s": round(part_eps, 2),
+        "plain_gpu_evals_per_s": round(plain_eps, 2),
+        "lnL": lnl0,
+        "dlnL_abs": dl_abs,
+        "dlnL_rel": dl_rel,
+        "kernel_ms": {"pmat": round(float(tm[0]), 4), "traverse": round(trav_ms, 4),
+                      "finalize": round(float(tm[2]), 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes": traffic, "node_updates": nodes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1184 @@
+// Host engine of libcrux_amd: device context, trees with device-resident CLV
+// pools, the likelihood planner and batched execution.
+//
+// Planner semantics follow Lik._plan/_planRecurse (pkg/Crux/Tree/Lik.pyx:
+// 1427-1601): every directed edge ("ring") has, per polarity, a cached CLV
+// with a validity flag and a cache key (parent ring, number of siblings,
+// branch length).  A key mismatch invalidates the parent; a recomputed CLV
+// invalidates its parent in turn; the root CL is recomputed whenever any of
+// its children changed.  Unlike the reference, all child steps of one parent
+// become ONE fused node update (children multiplied in registers), and the
+// plan is executed for all sites by a single device launch.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "crux_lik.h"
+#include "device.h"
+#include "internal.h"
+
+namespace crux {
+
+int traverse_cmax(int K, int C);
+int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
+                    int blk0, int nblocks, hipStream_t stream);
+void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream);
+void launch_finalize(const Entry *entries, int nentries, hipStream_t stream);
+
+static thread_local char g_errbuf[512] = "";
+
+int set_error(int status, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_errbuf, sizeof g_errbuf, fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t err_ = (expr);                                                      \
+        if (err_ != hipSuccess)                                                        \
+            return set_error(err_ == hipErrorOutOfMemory ? CRUX_ENOMEM : CRUX_EDEVICE, \
+                             "%s: %s", #expr, hipGetErrorString(err_));                \
+    } while (0)
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace crux
+
+using namespace crux;
+
+// ---------------------------------------------------------------------------
+// Context.
+
+struct crux_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int mode = 0;
+    bool timing = false;
+    hipEvent_t ev[4] = {};
+    double last_ms[4] = {0, 0, 0, 0};
+    uint64_t last_bytes = 0;
+    int last_nodes = 0;
+    // Staging: one pinned host buffer mirrored by one device buffer.
+    char *h_plan = nullptr;
+    char *d_plan = nullptr;
+    size_t plan_cap = 0;
+    float *d_pmat = nullptr;       // P blocks and tip tables
+    size_t pmat_cap = 0;           // floats
+    double *d_partials = nullptr;  // [sum ntiles][2]
+    size_t partials_cap = 0;       // doubles
+    double *d_lnl = nullptr;
+    double *h_lnl = nullptr;
+    size_t lnl_cap = 0;
+    std::mutex mu;                 // one execution at a time per context
+};
+
+static int dev_alloc(crux_ctx *ctx, void **p, size_t bytes)
+{
+    if (ctx->budget && ctx->used + bytes > ctx->budget)
+        return set_error(CRUX_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu bytes",
+                         (unsigned long long)ctx->used, (unsigned long long)bytes,
+                         (unsigned long long)ctx->budget);
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(p, bytes ? bytes : 16));
+    ctx->used += bytes;
+    return CRUX_OK;
+}
+
+static void dev_free(crux_ctx *ctx, void *p, size_t bytes)
+{
+    if (p) {
+        (void)hipFree(p);
+        ctx->used -= bytes;
+    }
+}
+
+template <typename T>
+static int grow(crux_ctx *ctx, T **p, size_t *cap, size_t need)
+{
+    if (need <= *cap)
+        return CRUX_OK;
+    size_t ncap = std::max(need, *cap * 2);
+    dev_free(ctx, *p, *cap * sizeof(T));
+    *p = nullptr;
+    *cap = 0;
+    int rc = dev_alloc(ctx, (void **)p, ncap * sizeof(T));
+    if (rc)
+        return rc;
+    *cap = ncap;
+    return CRUX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Trees and likelihoods.
+
+namespace {
+
+struct RingCache {
+    int32_t slot = -1;
+    bool valid = false;
+    int32_t parent = INT32_MIN;   // ring id of the parent CL, kRootKey for the root CL
+    int32_t nsibs = -1;
+    double elen = NAN;
+};
+
+constexpr int32_t kRootKey = -1;
+
+struct Arena {
+    int C = 0;
+    size_t slot_bytes = 0;   // CLV [S][C][K] floats + exps [S] int32, 256-aligned
+    size_t clv_bytes = 0;
+    int chunk_slots = 0;
+    std::vector<char *> chunks;
+    std::vector<int32_t> free_slots;
+    int32_t nslots = 0;
+
+    float *clv(int32_t slot) const
+    {
+        return (float *)(chunks[slot / chunk_slots] + (size_t)(slot % chunk_slots) * slot_bytes);
+    }
+    int32_t *exps(int32_t slot) const
+    {
+        return (int32_t *)((char *)clv(slot) + clv_bytes);
+    }
+};
+
+struct PlanChild {
+    int kind;
+    int32_t ring;     // internal/tip: child ring
+    int32_t taxon;    // tip: alignment row
+    int32_t slot;     // internal/ident: CLV slot
+    double len;
+};
+
+struct PlanOp {
+    int32_t parent_slot;   // -1: root reduce
+    std::vector<PlanChild> ch;
+};
+
+}  // namespace
+
+struct crux_tree {
+    crux_ctx *ctx = nullptr;
+    int K = 0, S = 0, ntaxa = 0, ncodes = 0;
+    uint8_t *d_codes = nullptr;
+    double *d_codevecs = nullptr;
+    uint32_t *d_weights = nullptr;
+    bool has_topology = false;
+    int nnodes = 0, nedges = 0;
+    std::vector<int32_t> ring_node, ring_next, node_ring, node_taxon, degree;
+    std::vector<double> edge_len;
+    std::vector<RingCache> cache[2];
+    Arena arena[2];
+    crux_lik *owner[2] = {nullptr, nullptr};
+};
+
+struct crux_lik {
+    crux_tree *tree = nullptr;
+    int pol = 0;
+    int C = 0;
+    bool have_model = false;
+    bool invalidate = true;
+    bool root_valid = false;
+    int32_t root_tmp_slot = -1;    // scratch CL for roots of degree > kOpMaxChild
+    double *d_model = nullptr;     // [C][model_stride(K)]
+    size_t model_bytes = 0;
+    double *d_piw = nullptr;       // [C][K]
+    double *d_site = nullptr;      // [S]
+    double last_lnl = 0.0;
+    bool have_lnl = false;
+    // plan of the current execution
+    std::vector<PlanOp> ops;
+};
+
+static void arena_reset(crux_tree *t, int pol)
+{
+    Arena &a = t->arena[pol];
+    for (char *c : a.chunks)
+        dev_free(t->ctx, c, (size_t)a.chunk_slots * a.slot_bytes);
+    a.chunks.clear();
+    a.free_slots.clear();
+    a.nslots = 0;
+    a.C = 0;
+    for (RingCache &rc : t->cache[pol]) {
+        rc.slot = -1;
+        rc.valid = false;
+    }
+}
+
+static int arena_get(crux_tree *t, int pol, int32_t *slot)
+{
+    Arena &a = t->arena[pol];
+    if (a.free_slots.empty()) {
+        if (a.chunk_slots == 0) {
+            // About 64 MiB per allocation, at least one slot.
+            a.chunk_slots = (int)std::max<size_t>(1, ((size_t)64 << 20) / a.slot_bytes);
+            a.chunk_slots = std::min(a.chunk_slots, std::max(1, t->ntaxa));
+        }
+        char *p = nullptr;
+        int rc = dev_alloc(t->ctx, (void **)&p, (size_t)a.chunk_slots * a.slot_bytes);
+        if (rc)
+            return rc;
+        a.chunks.push_back(p);
+        for (int i = a.chunk_slots - 1; i >= 0; i--)
+            a.free_slots.push_back(a.nslots + i);
+        a.nslots += a.chunk_slots;
+    }
+    *slot = a.free_slots.back();
+    a.free_slots.pop_back();
+    return CRUX_OK;
+}
+
+static void arena_put(crux_tree *t, int pol, int32_t slot)
+{
+    if (slot >= 0)
+        t->arena[pol].free_slots.push_back(slot);
+}
+
+static int arena_layout(crux_tree *t, int pol, int C)
+{
+    Arena &a = t->arena[pol];
+    if (a.C == C)
+        return CRUX_OK;
+    arena_reset(t, pol);
+    a.C = C;
+    a.clv_bytes = align_up((size_t)t->S * C * t->K * sizeof(float), 256);
+    a.slot_bytes = a.clv_bytes + align_up((size_t)t->S * sizeof(int32_t), 256);
+    a.chunk_slots = 0;
+    return CRUX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Planner (Lik._plan / _planRecurse restated over ring arrays).
+
+namespace {
+
+struct Planner {
+    crux_lik *L;
+    crux_tree *t;
+    int pol;
+    bool inval;
+    std::vector<PlanOp> &ops;
+    int rc = CRUX_OK;
+
+    bool &valid_of(int32_t key)
+    {
+        return key == kRootKey ? L->root_valid : t->cache[pol][key].valid;
+    }
+
+    void flush(int32_t ring)
+    {
+        // Drop a CL this plan will not use (Lik.pyx:1492-1496).
+        RingCache &c = t->cache[pol][ring];
+        if (t->degree[t->ring_node[ring]] > 1 && c.slot >= 0) {
+            arena_put(t, pol, c.slot);
+            c.slot = -1;
+        }
+        c.valid = false;
+        c.parent = INT32_MIN;
+    }
+
+    PlanChild child_of(int32_t ring, double len)
+    {
+        PlanChild pc{};
+        pc.ring = ring;
+        pc.len = len;
+        int node = t->ring_node[ring];
+        if (t->degree[node] == 1) {
+            pc.kind = kChildTip;
+            pc.taxon = t->node_taxon[node];
+            pc.slot = -1;
+        } else {
+            pc.kind = kChildInternal;
+            pc.taxon = -1;
+            pc.slot = t->cache[pol][ring].slot;
+        }
+        return pc;
+    }
+
+    // Emit the fused node update(s) for target slot (or the root reduce when
+    // target == -1) over the given children.
+    void emit(int32_t target, std::vector<PlanChild> &ch)
+    {
+        size_t i = 0;
+        int32_t acc_slot = target;
+        if (target < 0 && ch.size() > (size_t)kOpMaxChild) {
+            if (L->root_tmp_slot < 0) {
+                rc = arena_get(t, pol, &L->root_tmp_slot);
+                if (rc)
+                    return;
+            }
+            acc_slot = L->root_tmp_slot;
+        }
+        bool first = true;
+        while (i < ch.size()) {
+            PlanOp op;
+            size_t room = kOpMaxChild;
+            if (!first) {
+                PlanChild id{};
+                id.kind = kChildIdent;
+                id.ring = -1;
+                id.taxon = -1;
+                id.slot = acc_slot;
+                id.len = 0.0;
+                op.ch.push_back(id);
+                room -= 1;
+            }
+            while (room > 0 && i < ch.size()) {
+                op.ch.push_back(ch[i++]);
+                room--;
+            }
+            bool last = (i == ch.size());
+            op.parent_slot = last ? target : acc_slot;
+            ops.push_back(std::move(op));
+            first = false;
+        }
+    }
+
+    void recurse(int32_t ring, int32_t parent_key, int nsibs, double elen)
+    {
+        if (rc)
+            return;
+        const int node = t->ring_node[ring];
+        const int deg = t->degree[node];
+        RingCache &me = t->cache[pol][ring];
+        if (deg == 1) {
+            if (t->node_taxon[node] == -1) {
+                rc = set_error(CRUX_ETAXON, "Leaf node missing taxon");
+                return;
+            }
+            if (t->node_taxon[node] < 0 || t->node_taxon[node] >= t->ntaxa) {
+                rc = set_error(CRUX_ETAXON, "Taxon missing from alignment's taxa map");
+                return;
+            }
+        } else if (me.slot < 0) {
+            rc = arena_get(t, pol, &me.slot);
+            if (rc)
+                return;
+            me.valid = false;
+        }
+        for (int32_t r = t->ring_next[ring]; r != ring; r = t->ring_next[r]) {
+            if (inval)
+                flush(r);
+            recurse(r ^ 1, ring, deg, t->edge_len[r >> 1]);
+            if (rc)
+                return;
+        }
+        if (inval || me.parent != parent_key || me.nsibs != nsibs || me.elen != elen) {
+            valid_of(parent_key) = false;
+            me.parent = parent_key;
+            me.nsibs = nsibs;
+            me.elen = elen;
+        }
+        if (deg != 1 && (inval || !me.valid)) {
+            std::vector<PlanChild> ch;
+            for (int32_t r = t->ring_next[ring]; r != ring; r = t->ring_next[r]) {
+                double len = t->edge_len[r >> 1];
+                if (len < 0.0) {
+                    rc = set_error(CRUX_ENEGLEN, "Negative branch length");
+                    return;
+                }
+                ch.push_back(child_of(r ^ 1, len));
+            }
+            emit(me.slot, ch);
+            me.valid = true;
+            valid_of(parent_key) = false;
+        }
+    }
+
+    void plan(int root)
+    {
+        const int32_t ring = t->node_ring[root];
+        const int deg = t->degree[root];
+        if (ring < 0) {
+            rc = set_error(CRUX_EINVAL, "root node %d has no edges", root);
+            return;
+        }
+        if (deg == 1) {
+            // Leaf root: its neighbour over the edge, then the leaf itself
+            // over a zero-length branch (Lik.pyx:1551-1568).
+            recurse(ring ^ 1, kRootKey, 2, t->edge_len[ring >> 1]);
+            recurse(ring, kRootKey, 2, 0.0);
+            if (rc)
+                return;
+            if (inval || !L->root_valid) {
+                double len = t->edge_len[ring >> 1];
+                if (len < 0.0) {
+                    rc = set_error(CRUX_ENEGLEN, "Negative branch length");
+                    return;
+                }
+                std::vector<PlanChild> ch{child_of(ring ^ 1, len), child_of(ring, 0.0)};
+                emit(-1, ch);
+                L->root_valid = true;
+            }
+        } else {
+            int32_t r = ring;
+            do {
+                if (inval)
+                    flush(r);
+                recurse(r ^ 1, kRootKey, deg, t->edge_len[r >> 1]);
+                if (rc)
+                    return;
+                r = t->ring_next[r];
+            } while (r != ring);
+            if (inval || !L->root_valid) {
+                std::vector<PlanChild> ch;
+                r = ring;
+                do {
+                    double len = t->edge_len[r >> 1];
+                    if (len < 0.0) {
+                        rc = set_error(CRUX_ENEGLEN, "Negative branch length");
+                        return;
+                    }
+                    ch.push_back(child_of(r ^ 1, len));
+                    r = t->ring_next[r];
+                } while (r != ring);
+                emit(-1, ch);
+                L->root_valid = true;
+            }
+        }
+    }
+};
+
+}  // namespace
+
+static int plan_lik(crux_lik *L, int root)
+{
+    crux_tree *t = L->tree;
+    L->ops.clear();
+    if (!t->has_topology)
+        return set_error(CRUX_ESTATE, "tree has no topology");
+    if (!L->have_model)
+        return set_error(CRUX_ESTATE, "no model set");
+    if (root < 0 || root >= t->nnodes || t->node_ring[root] < 0)
+        return set_error(CRUX_EINVAL, "bad root node %d", root);
+    // Another lik of this polarity owned the CLVs: everything is stale.
+    bool inval = L->invalidate || t->owner[L->pol] != L;
+    if (t->arena[L->pol].C != L->C) {
+        arena_layout(t, L->pol, L->C);
+        L->root_tmp_slot = -1;
+        inval = true;
+    }
+    t->owner[L->pol] = L;
+    Planner p{L, t, L->pol, inval, L->ops};
+    if (inval)
+        L->root_valid = false;
+    p.plan(root);
+    if (p.rc) {
+        // Leave everything of this polarity invalid; the next call replans.
+        L->invalidate = true;
+        L->ops.clear();
+        return p.rc;
+    }
+    L->invalidate = false;
+    return CRUX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Execution of a batch of planned likelihoods.
+
+static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
+{
+    // Classes by (K, CM) for the traversal launches.
+    struct Item {
+        crux_lik *L;
+        int idx;    // output index
+        int CM;
+    };
+    std::vector<Item> items;
+    for (size_t i = 0; i < liks.size(); i++) {
+        crux_lik *L = liks[i];
+        if (L->ops.empty())
+            continue;
+        int cm = traverse_cmax(L->tree->K, L->C);
+        if (cm < 0)
+            return set_error(CRUX_EINVAL, "unsupported state/component count K=%d C=%d",
+                             L->tree->K, L->C);
+        items.push_back({L, (int)i, cm});
+    }
+    ctx->last_bytes = 0;
+    ctx->last_nodes = 0;
+    if (items.empty()) {
+        for (size_t i = 0; i < liks.size(); i++)
+            out[i] = liks[i]->last_lnl;
+        return CRUX_OK;
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+        if (a.L->tree->K != b.L->tree->K)
+            return a.L->tree->K < b.L->tree->K;
+        return a.CM < b.CM;
+    });
+
+    const bool plain = ctx->mode == 1;
+    // Sizes.
+    size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0;
+    size_t shmem = 0;
+    for (Item &it : items) {
+        crux_lik *L = it.L;
+        crux_tree *t = L->tree;
+        const int K = t->K, C = L->C;
+        nops += L->ops.size();
+        if (plain)
+            nplain += L->ops.size();
+        for (PlanOp &op : L->ops)
+            for (PlanChild &c : op.ch) {
+                if (c.kind == kChildInternal) {
+                    njobs++;
+                    pm_floats += align_up((size_t)C * K * K, 64);
+                    shmem = std::max(shmem, (size_t)C * K * sizeof(double));
+                } else if (c.kind == kChildTip) {
+                    njobs++;
+                    pm_floats += align_up((size_t)t->ncodes * C * K, 64);
+                    shmem = std::max(shmem, ((size_t)C * K + (size_t)C * K * K) * sizeof(double));
+                }
+            }
+        tiles += (t->S + kTile - 1) / kTile;
+    }
+    const size_t nent = items.size() + nplain;
+    const size_t off_ent = 0;
+    const size_t off_ops = align_up(off_ent + nent * sizeof(Entry), 256);
+    const size_t off_jobs = align_up(off_ops + nops * sizeof(Op), 256);
+    const size_t off_blk = align_up(off_jobs + njobs * sizeof(PJob), 256);
+    const size_t nblk_total = plain ? tiles * 0 : tiles;
+    size_t blk_count = 0;
+    for (Item &it : items) {
+        size_t nt = (it.L->tree->S + kTile - 1) / kTile;
+        blk_count += plain ? nt * (it.L->ops.size() + 1) : nt;
+    }
+    (void)nblk_total;
+    const size_t plan_bytes = align_up(off_blk + blk_count * sizeof(int32_t), 256);
+    if (shmem > 64 * 1024)
+        return set_error(CRUX_EINVAL, "P-matrix job too large for LDS");
+
+    if (plan_bytes > ctx->plan_cap) {
+        size_t ncap = std::max(plan_bytes, ctx->plan_cap * 2);
+        if (ctx->h_plan)
+            (void)hipHostFree(ctx->h_plan);
+        dev_free(ctx, ctx->d_plan, ctx->plan_cap);
+        ctx->h_plan = nullptr;
+        ctx->d_plan = nullptr;
+        ctx->plan_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&ctx->h_plan, ncap, hipHostMallocDefault));
+        int rc = dev_alloc(ctx, (void **)&ctx->d_plan, ncap);
+        if (rc)
+            return rc;
+        ctx->plan_cap = ncap;
+    }
+    int rc = grow(ctx, &ctx->d_pmat, &ctx->pmat_cap, std::max<size_t>(pm_floats, 64));
+    if (rc)
+        return rc;
+    rc = grow(ctx, &ctx->d_partials, &ctx->partials_cap, std::max<size_t>(2 * tiles, 2));
+    if (rc)
+        return rc;
+    if (liks.size() > ctx->lnl_cap) {
+        size_t ncap = std::max(liks.size(), ctx->lnl_cap * 2);
+        if (ctx->h_lnl)
+            (void)hipHostFree(ctx->h_lnl);
+        dev_free(ctx, ctx->d_lnl, ctx->lnl_cap * sizeof(double));
+        ctx->h_lnl = nullptr;
+        ctx->d_lnl = nullptr;
+        ctx->lnl_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&ctx->h_lnl, ncap * sizeof(double), hipHostMallocDefault));
+        rc = dev_alloc(ctx, (void **)&ctx->d_lnl, ncap * sizeof(double));
+        if (rc)
+            return rc;
+        ctx->lnl_cap = ncap;
+    }
+
+    char *H = ctx->h_plan, *D = ctx->d_plan;
+    Entry *hent = (Entry *)(H + off_ent);
+    Op *hops = (Op *)(H + off_ops);
+    PJob *hjobs = (PJob *)(H + off_jobs);
+    int32_t *hblk = (int32_t *)(H + off_blk);
+    const Entry *dent = (const Entry *)(D + off_ent);
+    const Op *dops = (const Op *)(D + off_ops);
+    const PJob *djobs = (const PJob *)(D + off_jobs);
+    const int32_t *dblk = (const int32_t *)(D + off_blk);
+
+    size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0;
+    uint64_t traffic = 0;
+    int nodes = 0;
+    // Fused launches: groups of consecutive items of one (K, CM) class.
+    struct Launch {
+        int K, CM;
+        size_t blk0, nblk;
+    };
+    std::vector<Launch> launches;
+    std::vector<size_t> main_entry(items.size());
+
+    for (size_t ii = 0; ii < items.size(); ii++) {
+        crux_lik *L = items[ii].L;
+        crux_tree *t = L->tree;
+        const int K = t->K, C = L->C;
+        const Arena &A = t->arena[L->pol];
+        const size_t ntl = (t->S + kTile - 1) / kTile;
+        const size_t op0 = io;
+        const uint64_t clv_b = (uint64_t)t->S * C * K * 4, exp_b = (uint64_t)t->S * 4;
+        for (PlanOp &pop : L->ops) {
+            Op &op = hops[io++];
+            memset(&op, 0, sizeof op);
+            op.nchild = (int32_t)pop.ch.size();
+            if (pop.parent_slot >= 0) {
+                op.parent = A.clv(pop.parent_slot);
+                op.parent_exp = A.exps(pop.parent_slot);
+                traffic += clv_b + exp_b;
+                nodes++;
+            } else {
+                traffic += 4ull * t->S + 8ull * t->S;   // weights in, site lnL out
+            }
+            for (size_t j = 0; j < pop.ch.size(); j++) {
+                PlanChild &pc = pop.ch[j];
+                OpChild &oc = op.ch[j];
+                op.kinds |= pc.kind << (2 * j);
+                if (pc.kind == kChildTip) {
+                    oc.codes = t->d_codes + (size_t)pc.taxon * t->S;
+                    traffic += (uint64_t)t->S;
+                } else {
+                    oc.clv = A.clv(pc.slot);
+                    oc.exps = A.exps(pc.slot);
+                    traffic += clv_b + exp_b;
+                }
+                if (pc.kind != kChildIdent) {
+                    PJob &jb = hjobs[ij++];
+                    jb.len = pc.len;
+                    jb.model = L->d_model;
+                    jb.codevecs = t->d_codevecs;
+                    jb.out = ctx->d_pmat + ipm;
+                    jb.tip = pc.kind == kChildTip;
+                    jb.C = C;
+                    jb.K = K;
+                    jb.ncodes = t->ncodes;
+                    oc.mat = jb.out;
+                    ipm += align_up(jb.tip ? (size_t)t->ncodes * C * K : (size_t)C * K * K, 64);
+                }
+            }
+        }
+        Entry base{};
+        base.ops = dops + op0;
+        base.nops = (int32_t)L->ops.size();
+        base.S = t->S;
+        base.C = C;
+        base.K = K;
+        base.ntiles = (int32_t)ntl;
+        base.piw = L->d_piw;
+        base.weights = t->d_weights;
+        base.site_out = L->d_site;
+        base.partials = ctx->d_partials + 2 * itile;
+        base.lnl_out = ctx->d_lnl + items[ii].idx;
+        itile += ntl;
+        if (!plain) {
+            base.tile0 = (int32_t)iblk;
+            main_entry[ii] = ient;
+            hent[ient] = base;
+            for (size_t b = 0; b < ntl; b++)
+                hblk[iblk++] = (int32_t)ient;
+            ient++;
+            if (launches.empty() || launches.back().K != K || launches.back().CM != items[ii].CM)
+                launches.push_back({K, items[ii].CM, iblk - ntl, 0});
+            launches.back().nblk += ntl;
+        } else {
+            // One launch per node update, each on a one-op entry.
+            main_entry[ii] = ient;
+            for (size_t o = 0; o < L->ops.size(); o++) {
+                Entry e1 = base;
+                e1.ops = dops + op0 + o;
+                e1.nops = 1;
+                e1.tile0 = (int32_t)iblk;
+                hent[ient] = e1;
+                for (size_t b = 0; b < ntl; b++)
+                    hblk[iblk++] = (int32_t)ient;
+                launches.push_back({K, items[ii].CM, iblk - ntl, ntl});
+                ient++;
+            }
+        }
+    }
+
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(D, H, plan_bytes, hipMemcpyHostToDevice, st));
+    if (ctx->timing)
+        HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    if (ij > 0)
+        launch_pmat(djobs, (int)ij, shmem, st);
+    HIP_TRY(hipGetLastError());
+    if (ctx->timing)
+        HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    for (Launch &ln : launches) {
+        if (ln.nblk == 0)
+            continue;
+        if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk, st) != 0)
+            return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
+        HIP_TRY(hipGetLastError());
+    }
+    if (ctx->timing)
+        HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    // Finalize: one wave per evaluation, over the evaluations' main entries.
+    // In plain mode the root op's entry carries the partials; the entries of
+    // one evaluation share partials and lnl_out, so finalize the last one.
+    if (!plain) {
+        launch_finalize(dent, (int)ient, st);
+    } else {
+        // Finalize reads ntiles/partials/lnl_out, identical across the
+        // one-op entries of an evaluation: launch over each evaluation's
+        // first entry.
+        for (size_t ii = 0; ii < items.size(); ii++)
+            launch_finalize(dent + main_entry[ii], 1, st);
+    }
+    HIP_TRY(hipGetLastError());
+    if (ctx->timing)
+        HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    HIP_TRY(hipMemcpyAsync(ctx->h_lnl, ctx->d_lnl, liks.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->timing) {
+        float a = 0, b = 0, c = 0, d = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[3]));
+        ctx->last_ms[0] = a;
+        ctx->last_ms[1] = b;
+        ctx->last_ms[2] = c;
+        ctx->last_ms[3] = d;
+    }
+    ctx->last_bytes = traffic;
+    ctx->last_nodes = nodes;
+    for (Item &it : items) {
+        it.L->last_lnl = ctx->h_lnl[it.idx];
+        it.L->have_lnl = true;
+    }
+    for (size_t i = 0; i < liks.size(); i++)
+        out[i] = liks[i]->last_lnl;
+    return CRUX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI.
+
+extern "C" {
+
+const char *crux_strerror(int status)
+{
+    switch (status) {
+    case CRUX_OK: return "ok";
+    case CRUX_EINVAL: return "invalid argument";
+    case CRUX_ENOMEM: return "out of device memory or HBM budget";
+    case CRUX_ENEGLEN: return "negative branch length";
+    case CRUX_EDECOMP: return "eigendecomposition failed";
+    case CRUX_EDEVICE: return "device error";
+    case CRUX_ETAXON: return "leaf taxon error";
+    case CRUX_ESTATE: return "call out of order";
+    default: return "unknown status";
+    }
+}
+
+const char *crux_last_error(void) { return g_errbuf; }
+
+int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
+{
+    if (!out)
+        return set_error(CRUX_EINVAL, "out is NULL");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return set_error(CRUX_EINVAL, "device %d of %d", device, n);
+    crux_ctx *ctx = new crux_ctx;
+    ctx->device = device;
+    ctx->budget = hbm_budget;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; i++)
+        e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        delete ctx;
+        return set_error(CRUX_EDEVICE, "context setup: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return CRUX_OK;
+}
+
+int crux_ctx_destroy(crux_ctx *ctx)
+{
+    if (!ctx)
+        return CRUX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->h_plan)
+        (void)hipHostFree(ctx->h_plan);
+    if (ctx->h_lnl)
+        (void)hipHostFree(ctx->h_lnl);
+    dev_free(ctx, ctx->d_plan, ctx->plan_cap);
+    dev_free(ctx, ctx->d_pmat, ctx->pmat_cap * sizeof(float));
+    dev_free(ctx, ctx->d_partials, ctx->partials_cap * sizeof(double));
+    dev_free(ctx, ctx->d_lnl, ctx->lnl_cap * sizeof(double));
+    for (int i = 0; i < 4; i++)
+        if (ctx->ev[i])
+            (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CRUX_OK;
+}
+
+uint64_t crux_ctx_bytes(const crux_ctx *ctx) { return ctx ? ctx->used : 0; }
+
+int crux_ctx_set_mode(crux_ctx *ctx, int mode)
+{
+    if (!ctx || mode < 0 || mode > 1)
+        return set_error(CRUX_EINVAL, "bad mode");
+    ctx->mode = mode;
+    return CRUX_OK;
+}
+
+int crux_ctx_set_timing(crux_ctx *ctx, int on)
+{
+    if (!ctx)
+        return set_error(CRUX_EINVAL, "ctx is NULL");
+    ctx->timing = on != 0;
+    return CRUX_OK;
+}
+
+int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4)
+{
+    if (!ctx || !ms4)
+        return set_error(CRUX_EINVAL, "NULL argument");
+    for (int i = 0; i < 4; i++)
+        ms4[i] = ctx->last_ms[i];
+    return CRUX_OK;
+}
+
+int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, int *nodes)
+{
+    if (!ctx)
+        return set_error(CRUX_EINVAL, "ctx is NULL");
+    if (bytes)
+        *bytes = ctx->last_bytes;
+    if (nodes)
+        *nodes = ctx->last_nodes;
+    return CRUX_OK;
+}
+
+int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_t *codes,
+                     int ncodes, const double *code_vecs, const uint32_t *weights,
+                     crux_tree **out)
+{
+    if (!ctx || !out || !codes || !code_vecs || !weights)
+        return set_error(CRUX_EINVAL, "NULL argument");
+    if (dim < 2 || dim > kMaxStates || nsites < 1 || ntaxa < 1 || ncodes < 1 || ncodes > 256)
+        return set_error(CRUX_EINVAL, "bad tree shape dim=%d nsites=%d ntaxa=%d ncodes=%d", dim,
+                         nsites, ntaxa, ncodes);
+    for (size_t i = 0; i < (size_t)ntaxa * nsites; i++)
+        if (codes[i] >= ncodes)
+            return set_error(CRUX_EINVAL, "code %d out of range at %zu", codes[i], i);
+    crux_tree *t = new crux_tree;
+    t->ctx = ctx;
+    t->K = dim;
+    t->S = nsites;
+    t->ntaxa = ntaxa;
+    t->ncodes = ncodes;
+    int rc = dev_alloc(ctx, (void **)&t->d_codes, (size_t)ntaxa * nsites);
+    if (!rc)
+        rc = dev_alloc(ctx, (void **)&t->d_codevecs, sizeof(double) * ncodes * dim);
+    if (!rc)
+        rc = dev_alloc(ctx, (void **)&t->d_weights, sizeof(uint32_t) * nsites);
+    if (rc) {
+        crux_tree_destroy(t);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(t->d_codes, codes, (size_t)ntaxa * nsites, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(t->d_codevecs, code_vecs, sizeof(double) * ncodes * dim,
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(t->d_weights, weights, sizeof(uint32_t) * nsites, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        crux_tree_destroy(t);
+        return set_error(CRUX_EDEVICE, "tree upload: %s", hipGetErrorString(e));
+    }
+    *out = t;
+    return CRUX_OK;
+}
+
+int crux_tree_destroy(crux_tree *t)
+{
+    if (!t)
+        return CRUX_OK;
+    crux_ctx *ctx = t->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    arena_reset(t, 0);
+    arena_reset(t, 1);
+    dev_free(ctx, t->d_codes, (size_t)t->ntaxa * t->S);
+    dev_free(ctx, t->d_codevecs, sizeof(double) * t->ncodes * t->K);
+    dev_free(ctx, t->d_weights, sizeof(uint32_t) * t->S);
+    for (int p = 0; p < 2; p++)
+        if (t->owner[p])
+            t->owner[p]->tree = nullptr;
+    delete t;
+    return CRUX_OK;
+}
+
+int crux_tree_set_topology(crux_tree *t, int nnodes, int nedges, const int32_t *ring_node,
+                           const int32_t *ring_next, const int32_t *node_ring,
+                           const int32_t *node_taxon, const double *edge_len)
+{
+    if (!t || nnodes < 1 || nedges < 0 || !ring_node || !ring_next || !node_ring ||
+        !node_taxon || (nedges > 0 && !edge_len))
+        return set_error(CRUX_EINVAL, "bad topology arguments");
+    const int nr = 2 * nedges;
+    std::vector<int32_t> degree(nnodes, 0);
+    for (int r = 0; r < nr; r++) {
+        if (ring_node[r] < -1 || ring_node[r] >= nnodes || ring_next[r] < -1 ||
+            ring_next[r] >= nr || (ring_node[r] >= 0) != (ring_next[r] >= 0) ||
+            (ring_node[r] >= 0) != (ring_node[r ^ 1] >= 0))
+            return set_error(CRUX_EINVAL, "bad ring %d", r);
+        if (ring_node[r] >= 0) {
+            if (ring_node[ring_next[r]] != ring_node[r])
+                return set_error(CRUX_EINVAL, "ring %d: next ring on another node", r);
+            degree[ring_node[r]]++;
+        }
+    }
+    for (int n = 0; n < nnodes; n++) {
+        if (node_ring[n] < -1 || node_ring[n] >= nr ||
+            (node_ring[n] >= 0 && ring_node[node_ring[n]] != n))
+            return set_error(CRUX_EINVAL, "bad node ring %d", n);
+        // Every ring of the node must be reachable around its cycle.
+        if (node_ring[n] >= 0) {
+            int cnt = 0;
+            int32_t r = node_ring[n];
+            do {
+                r = ring_next[r];
+                if (++cnt > degree[n])
+                    return set_error(CRUX_EINVAL, "ring cycle of node %d is broken", n);
+            } while (r != node_ring[n]);
+            if (cnt != degree[n])
+                return set_error(CRUX_EINVAL, "ring cycle of node %d is broken", n);
+        }
+    }
+    t->nnodes = nnodes;
+    t->nedges = nedges;
+    t->ring_node.assign(ring_node, ring_node + nr);
+    t->ring_next.assign(ring_next, ring_next + nr);
+    t->node_ring.assign(node_ring, node_ring + nnodes);
+    t->node_taxon.assign(node_taxon, node_taxon + nnodes);
+    t->edge_len.assign(edge_len, edge_len + nedges);
+    t->degree = degree;
+    for (int p = 0; p < 2; p++) {
+        if ((int)t->cache[p].size() < nr)
+            t->cache[p].resize(nr);
+        else if ((int)t->cache[p].size() > nr) {
+            for (int r = nr; r < (int)t->cache[p].size(); r++)
+                arena_put(t, p, t->cache[p][r].slot);
+            t->cache[p].resize(nr);
+        }
+    }
+    t->has_topology = true;
+    return CRUX_OK;
+}
+
+int crux_tree_set_lengths(crux_tree *t, int n, const int32_t *edges, const double *lens)
+{
+    if (!t || n < 0 || (n > 0 && (!edges || !lens)))
+        return set_error(CRUX_EINVAL, "bad arguments");
+    for (int i = 0; i < n; i++) {
+        if (edges[i] < 0 || edges[i] >= t->nedges)
+            return set_error(CRUX_EINVAL, "edge %d out of range", edges[i]);
+        t->edge_len[edges[i]] = lens[i];
+    }
+    return CRUX_OK;
+}
+
+int crux_lik_create(crux_tree *t, int polarity, crux_lik **out)
+{
+    if (!t || !out || polarity < 0 || polarity > 1)
+        return set_error(CRUX_EINVAL, "bad arguments");
+    crux_lik *L = new crux_lik;
+    L->tree = t;
+    L->pol = polarity;
+    int rc = dev_alloc(t->ctx, (void **)&L->d_site, sizeof(double) * t->S);
+    if (rc) {
+        delete L;
+        return rc;
+    }
+    *out = L;
+    return CRUX_OK;
+}
+
+int crux_lik_destroy(crux_lik *L)
+{
+    if (!L)
+        return CRUX_OK;
+    crux_tree *t = L->tree;
+    if (t) {
+        crux_ctx *ctx = t->ctx;
+        (void)hipStreamSynchronize(ctx->stream);
+        if (t->owner[L->pol] == L)
+            t->owner[L->pol] = nullptr;
+        if (L->root_tmp_slot >= 0 && t->arena[L->pol].C == L->C)
+            arena_put(t, L->pol, L->root_tmp_slot);
+        dev_free(ctx, L->d_model, L->model_bytes);
+        dev_free(ctx, L->d_piw, sizeof(double) * L->C * t->K);
+        dev_free(ctx, L->d_site, sizeof(double) * t->S);
+    }
+    delete L;
+    return CRUX_OK;
+}
+
+int crux_lik_set_model(crux_lik *L, int nmodels, const double *evec, const double *evec_inv,
+                       const double *evals, const double *pi_norm, int ncomp,
+                       const int32_t *comp_model, const double *comp_weight,
+                       const double *comp_rate, int invalidate)
+{
+    if (!L || !L->tree || nmodels < 1 || ncomp < 1 || !evec || !evec_inv || !evals ||
+        !pi_norm || !comp_model || !comp_weight || !comp_rate)
+        return set_error(CRUX_EINVAL, "bad model arguments");
+    crux_tree *t = L->tree;
+    crux_ctx *ctx = t->ctx;
+    const int K = t->K, st = model_stride(K);
+    if (traverse_cmax(K, ncomp) < 0)
+        return set_error(CRUX_EINVAL, "unsupported component count %d for %d states", ncomp, K);
+    std::vector<double> blk((size_t)ncomp * st), piw((size_t)ncomp * K);
+    for (int c = 0; c < ncomp; c++) {
+        int m = comp_model[c];
+        if (m < 0 || m >= nmodels)
+            return set_error(CRUX_EINVAL, "component %d: bad model %d", c, m);
+        if (!(comp_rate[c] >= 0.0) || !std::isfinite(comp_rate[c]))
+            return set_error(CRUX_EINVAL, "component %d: bad rate", c);
+        double *b = &blk[(size_t)c * st];
+        memcpy(b, evec + (size_t)m * K * K, sizeof(double) * K * K);
+        memcpy(b + K * K, evec_inv + (size_t)m * K * K, sizeof(double) * K * K);
+        memcpy(b + 2 * K * K, evals + (size_t)m * K, sizeof(double) * K);
+        b[2 * K * K + K] = comp_rate[c];
+        for (int i = 0; i < K; i++)
+            piw[(size_t)c * K + i] = pi_norm[(size_t)m * K + i] * comp_weight[c];
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    size_t mb = blk.size() * sizeof(double);
+    if (mb != L->model_bytes || ncomp != L->C) {
+        dev_free(ctx, L->d_model, L->model_bytes);
+        dev_free(ctx, L->d_piw, sizeof(double) * L->C * K);
+        L->d_model = nullptr;
+        L->d_piw = nullptr;
+        L->model_bytes = 0;
+        int rc = dev_alloc(ctx, (void **)&L->d_model, mb);
+        if (!rc)
+            rc = dev_alloc(ctx, (void **)&L->d_piw, sizeof(double) * ncomp * K);
+        if (rc) {
+            L->have_model = false;
+            return rc;
+        }
+        L->model_bytes = mb;
+    }
+    HIP_TRY(hipMemcpy(L->d_model, blk.data(), mb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(L->d_piw, piw.data(), sizeof(double) * ncomp * K, hipMemcpyHostToDevice));
+    if (ncomp != L->C)
+        invalidate = 1;
+    L->C = ncomp;
+    L->have_model = true;
+    if (invalidate)
+        L->invalidate = true;
+    return CRUX_OK;
+}
+
+static int run_one(crux_lik *L, int root)
+{
+    crux_ctx *ctx = L->tree->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    int rc = plan_lik(L, root);
+    if (rc)
+        return rc;
+    std::vector<crux_lik *> v{L};
+    double out;
+    rc = execute(ctx, v, &out);
+    if (rc) {
+        L->invalidate = true;
+        return rc;
+    }
+    return CRUX_OK;
+}
+
+int crux_lik_lnl(crux_lik *L, int root, double *lnl)
+{
+    if (!L || !L->tree || !lnl)
+        return set_error(CRUX_EINVAL, "bad arguments");
+    int rc = run_one(L, root);
+    if (rc)
+        return rc;
+    *lnl = L->last_lnl;
+    return CRUX_OK;
+}
+
+int crux_lik_site_lnl(crux_lik *L, int root, double *out, int n)
+{
+    if (!L || !L->tree || !out || n < 0 || n > L->tree->S)
+        return set_error(CRUX_EINVAL, "bad arguments");
+    int rc = run_one(L, root);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpy(out, L->d_site, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return CRUX_OK;
+}
+
+int crux_lik_flush(crux_lik *L)
+{
+    if (!L || !L->tree)
+        return set_error(CRUX_EINVAL, "bad arguments");
+    crux_tree *t = L->tree;
+    if (t->owner[L->pol] == L) {
+        for (size_t r = 0; r < t->cache[L->pol].size(); r++) {
+            RingCache &c = t->cache[L->pol][r];
+            if (c.slot >= 0) {
+                arena_put(t, L->pol, c.slot);
+                c.slot = -1;
+            }
+            c.valid = false;
+            c.parent = INT32_MIN;
+        }
+    }
+    L->root_valid = false;
+    return CRUX_OK;
+}
+
+int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t *roots,
+                   double *out)
+{
+    if (!ctx || n < 0 || (n > 0 && (!liks || !roots || !out)))
+        return set_error(CRUX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    std::vector<crux_lik *> v(liks, liks + n);
+    // Each tree-polarity may appear once: the plan mutates its cache state.
+    for (int i = 0; i < n; i++) {
+        if (!v[i] || !v[i]->tree || v[i]->tree->ctx != ctx)
+            return set_error(CRUX_EINVAL, "lik %d: bad handle", i);
+        for (int j = 0; j < i; j++)
+            if (v[j]->tree == v[i]->tree && v[j]->pol == v[i]->pol)
+                return set_error(CRUX_EINVAL, "liks %d and %d share a tree polarity", j, i);
+    }
+    for (int i = 0; i < n; i++) {
+        int rc = plan_lik(v[i], roots[i]);
+        if (rc) {
+            for (int j = 0; j < i; j++)
+                v[j]->invalidate = true;
+            return set_error(rc, "lik %d: %s", i, crux_last_error());
+        }
+    }
+    int rc = execute(ctx, v, out);
+    if (rc)
+        for (crux_lik *L : v)
+            L->invalidate = true;
+    return rc;
+}
+
+}  // extern "C"

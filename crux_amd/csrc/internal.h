@@ -1,0 +1,21 @@
+// Internal declarations shared by the host sources of libcrux_amd.
+#pragma once
+
+#include <cstdarg>
+#include <cstdint>
+
+namespace crux {
+
+constexpr int kMaxStates = 64;   // largest state count the host math accepts
+constexpr int kMaxChild = 3;     // children fused into one node update
+
+// Record a formatted detail message for crux_last_error() and return status.
+int set_error(int status, const char *fmt, ...);
+
+int q_decomp(int n, const double *rtri, const double *pi, double *pi_norm,
+             double *q_norm, double *evec, double *evec_inv, double *evals);
+int pt(int n, const double *evec, const double *evec_inv, const double *evals,
+       double t, double *P);
+int gamma_rates(double alpha, int ncat, int median, double *cmult);
+
+}  // namespace crux

@@ -14,8 +14,8 @@ against (pkg/Crux/Tree/__init__.pyx:32-1076):
 Two additions serve the device mirror of the tree: every Node and Edge gets a
 stable integer id (rings are ``2*edge.id + side``), and the tree keeps a
 topology serial number ``sn`` and a branch-length serial number ``lsn`` plus
-the set of edges whose length changed, so a likelihood object can push only
-what changed to the device.
+per-consumer sets of edges whose length changed, so a likelihood object can
+push only what changed to the device.
 """
 
 from __future__ import annotations
@@ -143,6 +143,7 @@ class Edge:
                 prev.next = ring
             node.ring = ring
         self.tree.sn += 1
+        self.tree.tsn += 1
 
     def detach(self):
         for ring in (self.ring, self.ring.other):
@@ -160,6 +161,7 @@ class Edge:
                 ring.prev = ring
             ring.node = None
         self.tree.sn += 1
+        self.tree.tsn += 1
 
     def __repr__(self):
         return "<Edge %d %g>" % (self.id, self._length)
@@ -279,9 +281,10 @@ class Tree:
         self._next_edge_id = 0
         self._base = None
         self.rooted = rooted
-        self.sn = 0          # topology serial number
+        self.sn = 0          # serial number of topology or base changes
+        self.tsn = 0         # serial number of attach/detach only
         self.lsn = 0         # branch-length serial number
-        self.dirty_edges = set()
+        self._sinks = []     # sets that collect edges whose length changed
         self.aux = None
         self._cache_sn = -1
         self._nodes = []
@@ -316,7 +319,15 @@ class Tree:
 
     def _length_changed(self, edge):
         self.lsn += 1
-        self.dirty_edges.add(edge)
+        for sink in self._sinks:
+            sink.add(edge)
+
+    def length_sink(self):
+        """A set that from now on collects every edge whose length changes
+        (device mirrors drain it before each evaluation)."""
+        s = set()
+        self._sinks.append(s)
+        return s
 
     def _label(self, node, label):
         node.taxon = label
@@ -419,14 +430,14 @@ class Tree:
             ring = node.ring
             self.base = ring.other.node
             edge = ring.edge
-            removed = np.float32(edge.length)
+            removed = float(np.float32(edge.length))
             edge.detach()
             ring = node.ring
             edge = ring.edge
             other = ring.other.node
             edge.detach()
             edge.attach(self._base, other)
-            edge.length = float(edge.length + removed)
+            edge.length = edge.length + removed
             self._base.ring = self._base.ring.next
         else:
             self.base = node
@@ -550,7 +561,6 @@ class Tree:
         if self._base is not None:
             t._base = copy(self._base, None)
         t.sn += 1
-        t.dirty_edges.clear()
         return t
 
     # -- output --------------------------------------------------------------

@@ -1,0 +1,176 @@
+/*
+ * crux_lik.h -- C ABI of the MI355X tree-likelihood engine.
+ *
+ * The reference's likelihood sits behind a Python class (Lik,
+ * pkg/Crux/Tree/Lik.pxd:23-98) over a C core whose ABI is three functions on
+ * caller-owned host FP64 buffers (lib/Crux/Tree/CxLik.h:279-285):
+ *
+ *     bool CxLikQDecomp(int n, double *RTri, double *PiDiag,
+ *                       double *PiDiagNorm, double *qNorm,
+ *                       double *qEigVecCube, double *qEigVals);
+ *     void CxLikPt(int n, double *P, double *qEigVecCube, double *qEigVals,
+ *                  double v);
+ *     void CxLikExecute(CxtLik *lik);
+ *
+ * Host FP64 CLVs cannot stay device-resident, so this ABI replaces that
+ * struct-passing core with opaque handles whose state lives in HBM: a tree
+ * (alignment patterns + topology + a CLV pool with two polarity slots per
+ * directed edge) and a lik (one mixture model + one polarity over a tree).
+ * The Python Lik class (crux_amd/lik.py) keeps the reference's call surface
+ * on top of it.  All entry points return an int status (CRUX_OK == 0); a
+ * -inf lnL is a result, not an error.  A handle is used by one thread at a
+ * time; calls on different trees of one context may come from different
+ * threads.
+ */
+#ifndef CRUX_LIK_H
+#define CRUX_LIK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CRUX_OK = 0,
+    CRUX_EINVAL = 1,    /* bad argument or shape */
+    CRUX_ENOMEM = 2,    /* device allocation failed or HBM budget exceeded */
+    CRUX_ENEGLEN = 3,   /* negative branch length (Lik.pyx:1423-1424) */
+    CRUX_EDECOMP = 4,   /* eigendecomposition failed (Lik.pyx:472-475) */
+    CRUX_EDEVICE = 5,   /* HIP runtime error */
+    CRUX_ETAXON = 6,    /* leaf without a usable taxon (Lik.pyx:1449-1464) */
+    CRUX_ESTATE = 7     /* call out of order (no model, no topology, ...) */
+};
+
+typedef struct crux_ctx crux_ctx;
+typedef struct crux_tree crux_tree;
+typedef struct crux_lik crux_lik;
+
+/* Text of a status code, and the detail message of the calling thread's
+ * last failure. */
+const char *crux_strerror(int status);
+const char *crux_last_error(void);
+
+/* ---- FP64 model setup on the host ------------------------------------- */
+
+/* Replaces CxLikQDecomp (CxLik.c:177-250).  rtri holds the n(n-1)/2
+ * exchangeabilities in the reference's upper-triangle order (CxLik.c:36-53,
+ * AC AG AT CG CT GT for DNA); pi the n (unnormalised) frequencies.  Outputs:
+ * pi_norm[n], *q_norm (1 / mean rate of the unscaled Q, +inf for Q == 0),
+ * evec[n*n] (V, row-major, columns are right eigenvectors), evec_inv[n*n]
+ * (V^-1) and evals[n], with Q = V diag(evals) V^-1. */
+int crux_q_decomp(int n, const double *rtri, const double *pi, double *pi_norm,
+                  double *q_norm, double *evec, double *evec_inv, double *evals);
+
+/* Replaces CxLikPt (CxLik.c:258-292): P = max(0, V exp(evals*t) V^-1),
+ * row-major, row = parent state. */
+int crux_pt(int n, const double *evec, const double *evec_inv,
+            const double *evals, double t, double *P);
+
+/* Gamma rate categories, replacing the CxMath use in Lik.setAlpha
+ * (Lik.pyx:1196-1257; CxMath.pxd:125-368): cmult[ncat] with mean 1 from
+ * category means (median == 0) or medians (median != 0). */
+int crux_gamma_rates(double alpha, int ncat, int median, double *cmult);
+
+/* ---- device context --------------------------------------------------- */
+
+/* One context per device.  hbm_budget caps the bytes the context allocates
+ * (0 = no cap).  The threaded CPU pool of the reference (CxLik.c:849-968)
+ * has no counterpart: sites map to GPU lanes. */
+int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out);
+int crux_ctx_destroy(crux_ctx *ctx);
+/* Bytes currently allocated by the context. */
+uint64_t crux_ctx_bytes(const crux_ctx *ctx);
+
+/* ---- trees -------------------------------------------------------------- */
+
+/* Alignment data of a tree: nsites patterns (integer weights) over ntaxa
+ * rows.  codes[ntaxa*nsites] indexes code_vecs[ncodes*dim], the 0/1 leaf
+ * vector of each distinct state set (gap already mapped to "any",
+ * Lik.pyx:1466-1475).  Replaces the leaf CL construction of
+ * Lik._planRecurse and charFreqs (Lik.pyx:313-314,1446-1475). */
+int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa,
+                     const uint8_t *codes, int ncodes, const double *code_vecs,
+                     const uint32_t *weights, crux_tree **out);
+int crux_tree_destroy(crux_tree *tree);
+
+/* Topology as ring arrays (a ring is one end of an edge; ring r belongs to
+ * edge r/2 and its other end is r^1): ring_node[2*nedges], ring_next[2*nedges]
+ * (next ring around the same node, -1 if unused), node_ring[nnodes] (first
+ * ring of the node, -1 if none), node_taxon[nnodes] (alignment row, -1 for
+ * an internal node, -2 for a label missing from the alignment) and
+ * edge_len[nedges].  Ring and node ids must stay stable across calls so
+ * cached CLVs survive; cache validity is re-derived per lnL call exactly as
+ * Lik._planRecurse does (Lik.pyx:1500-1532). */
+int crux_tree_set_topology(crux_tree *tree, int nnodes, int nedges,
+                           const int32_t *ring_node, const int32_t *ring_next,
+                           const int32_t *node_ring, const int32_t *node_taxon,
+                           const double *edge_len);
+/* Update some branch lengths (Edge.length writes). */
+int crux_tree_set_lengths(crux_tree *tree, int n, const int32_t *edges,
+                          const double *lens);
+
+/* ---- likelihoods ---------------------------------------------------------- */
+
+/* A mixture model evaluated on a tree with CLV slots of one polarity
+ * (Lik.clone() makes the mate with the other polarity, Lik.pyx:835-883). */
+int crux_lik_create(crux_tree *tree, int polarity, crux_lik **out);
+int crux_lik_destroy(crux_lik *lik);
+
+/* Active mixture components (weightScaled != 0; zero-weight components are
+ * skipped everywhere, CxLik.c:337,357,604,618).  For component c of model
+ * comp_model[c]: weight comp_weight[c] (weightScaled) and time factor
+ * comp_rate[c] = cmult * rmult * wNorm (CxLik.c:344-345).  Per model m:
+ * evec/evec_inv/evals as from crux_q_decomp and pi_norm.  invalidate != 0
+ * drops every cached CLV of this polarity (Lik.prep/_plan invalidate flag,
+ * Lik.pyx:1639,1664,1278). */
+int crux_lik_set_model(crux_lik *lik, int nmodels, const double *evec,
+                       const double *evec_inv, const double *evals,
+                       const double *pi_norm, int ncomp,
+                       const int32_t *comp_model, const double *comp_weight,
+                       const double *comp_rate, int invalidate);
+
+/* lnL with root_node as the root (Lik.lnL, Lik.pyx:1687-1737): plans the
+ * dirty CLVs, runs them on the device, sums the pattern-weighted site lnLs
+ * with compensated FP64 accumulation.  Synchronous. */
+int crux_lik_lnl(crux_lik *lik, int root_node, double *lnl);
+
+/* Pattern-weighted site lnLs of the last evaluation at root_node
+ * (Lik.siteLnLs, Lik.pyx:1739-1763); evaluates first when dirty. */
+int crux_lik_site_lnl(crux_lik *lik, int root_node, double *out, int n);
+
+/* Drop this polarity's cached CLVs (Lik.flush, Lik.pyx:1765-1787). */
+int crux_lik_flush(crux_lik *lik);
+
+/* ---- batches ------------------------------------------------------------ */
+
+/* Evaluate n likelihoods in one fused device pass (one P-matrix launch, one
+ * traversal launch per state-count/component-count class, one reduction
+ * launch).  Each lik must be on a different tree-polarity, or repeat the
+ * same one with the same root.  out[i] receives lnL of liks[i] rooted at
+ * roots[i].  Synchronous. */
+int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks,
+                   const int32_t *roots, double *out);
+
+/* Execution switches of the context (bench and tests):
+ *   mode 0: fused (default) -- one traversal launch per batch;
+ *   mode 1: plain -- one launch per node update plus one for the root,
+ *           the "one call per eval, no fusion" comparison path. */
+int crux_ctx_set_mode(crux_ctx *ctx, int mode);
+
+/* Timing of the last evaluation on the device stream, in milliseconds:
+ * [0] P-matrix kernel, [1] traversal kernel(s), [2] reduction, [3] total.
+ * Only filled when timing was enabled with crux_ctx_set_timing(ctx, 1). */
+int crux_ctx_set_timing(crux_ctx *ctx, int on);
+int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4);
+
+/* Per-call algorithmic traffic of the last evaluation in bytes (CLV reads and
+ * writes, tip codes, scale exponents, weights) and the number of node
+ * updates it ran. */
+int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, int *nodes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRUX_LIK_H */

@@ -227,7 +227,20 @@ for j in range(4):
                   "evals": [{"root": None, "lnL": float(v), "dec": decimals(v),
                              "sig": sig}]})
 
+# Rendered trees after deroot() + canonize() (every script prints them with
+# %.7e lengths before evaluating).
+renders = []
+for name in ("Lik_b", "Lik_c", "Lik_e", "Lik_g", "Lik_j", "Lik_l", "Lik_m"):
+    src, exp = read(name + ".cx"), read(name + ".cx.exp")
+    outs = re.findall(r"^(\(.*;)$", exp, re.M)
+    ins = trees_list(src)
+    # Lik_e prints each tree once; Lik_h prints every tree per alignment.
+    taxa = re.findall(r"^>(\S+)", fasta_strs(src)[0], re.M)
+    for tin, tout in zip(ins, outs):
+        renders.append({"case": name, "newick": tin, "render": tout,
+                        "taxa": taxa})
+
 with open(OUT, "w") as f:
     json.dump({"source": "reference test/Lik_*.cx and test/Lik_*.cx.exp",
-               "cases": cases}, f, indent=1)
+               "cases": cases, "renders": renders}, f, indent=1)
 print("wrote %d cases to %s" % (len(cases), OUT))

@@ -5,10 +5,12 @@ from crux_amd.alignment import Alignment
 from crux_amd.tree import Tree
 
 
-def build_tree(newick):
+def build_tree(newick, taxa=None):
+    """Parse, deroot and canonize by the alignment's taxa order, as every
+    reference Lik test does."""
     t = Tree(newick)
     t.deroot()
-    t.canonize()
+    t.canonize(taxa)
     return t
 
 
@@ -21,7 +23,7 @@ def apply_ops(lik, ops):
 def run_case(case, make_lik):
     """Yield (eval_spec, lnL, siteLnLs or None) for each evaluation."""
     aln = Alignment(case["fasta"])
-    tree = build_tree(case["tree"])
+    tree = build_tree(case["tree"], aln.taxa)
     lik = make_lik(tree, aln, **case.get("ctor", {}))
     apply_ops(lik, case["ops"])
     nodes = list(tree.nodes)
