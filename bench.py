@@ -40,12 +40,26 @@ def main():
     ap.add_argument("--partial-steps", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (the reference "
+                         "kernel for scaling HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the likelihood on a single "
               "device (see DESIGN.md for how the path shards); --gpus %d is not "
               "supported" % args.gpus, file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n, "copies": 5}))
+        return 0
 
     from crux_amd import _native as nat
     from crux_amd import synth
