@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--partial-steps", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stack-depth", type=int, default=None,
+                    help="4-state kernel LDS stack depth (2, 4, 8; default: tuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the reference "
                          "kernel for scaling HBM byte counters) and exit")
@@ -71,6 +73,8 @@ def main():
     log("problem: 128 taxa, %d patterns, %.1fs" % (S, time.time() - t0))
 
     ctx = nat.get_context(0)
+    if args.stack_depth is not None:
+        ctx.set_tuning(nat.CRUX_TUNE_STACK_DEPTH, args.stack_depth)
     lik = Lik(prob.tree, prob.alignment, ncat=4)
     synth.configure(lik, prob.model)
     root = prob.tree.base
@@ -86,7 +90,7 @@ def main():
     ctx.set_timing(True)
     full()
     tm = ctx.last_timing()
-    traffic, nodes = ctx.last_traffic()
+    traffic, formula_bytes, nodes = ctx.last_traffic()
     ctx.set_timing(False)
     t = time.perf_counter()
     for _ in range(args.steps):
@@ -193,7 +197,9 @@ def main():
                       "finalize": round(float(tm[2]), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes": traffic, "node_updates": nodes},
+                     "traffic": None, "algorithmic_bytes": traffic, "node_updates": nodes,
+                     "model_bytes": formula_bytes,
+                     "model_GBps": round(formula_bytes / (trav_ms * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libcrux_amd.so")
 
 CRUX_OK, CRUX_EINVAL, CRUX_ENOMEM, CRUX_ENEGLEN, CRUX_EDECOMP, CRUX_EDEVICE, \
     CRUX_ETAXON, CRUX_ESTATE = range(8)
+CRUX_TUNE_STACK_DEPTH = 1
 
 #: Every entry point the header declares, with (restype, argtypes).
 _vp = ctypes.c_void_p
@@ -48,9 +49,12 @@ SIGNATURES = {
     "crux_lik_flush": (ctypes.c_int, [_vp]),
     "crux_batch_lnl": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _i32p, _dp]),
     "crux_ctx_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "crux_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "crux_ctx_get_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "crux_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
     "crux_ctx_last_traffic": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_int)]),
 }
 
@@ -174,6 +178,14 @@ class Context:
     def set_mode(self, mode):
         check(lib().crux_ctx_set_mode(self.handle, int(mode)))
 
+    def set_tuning(self, key, value):
+        check(lib().crux_ctx_set_tuning(self.handle, int(key), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        check(lib().crux_ctx_get_tuning(self.handle, int(key), ctypes.byref(v)))
+        return v.value
+
     def set_timing(self, on):
         check(lib().crux_ctx_set_timing(self.handle, 1 if on else 0))
 
@@ -183,10 +195,14 @@ class Context:
         return out
 
     def last_traffic(self):
+        """(bytes moved, bytes under the all-children-from-HBM model, node
+        updates) of the last evaluation."""
         b = ctypes.c_uint64()
+        f = ctypes.c_uint64()
         n = ctypes.c_int()
-        check(lib().crux_ctx_last_traffic(self.handle, ctypes.byref(b), ctypes.byref(n)))
-        return b.value, n.value
+        check(lib().crux_ctx_last_traffic(self.handle, ctypes.byref(b), ctypes.byref(f),
+                                          ctypes.byref(n)))
+        return b.value, f.value, n.value
 
 
 _contexts = {}

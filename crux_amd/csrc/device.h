@@ -3,7 +3,9 @@
 // HBM layout (all FP32 CLVs are site-major like the reference's cLMat,
 // lib/Crux/Tree/CxLik.h:10-36, but FP32 with an int32 power-of-two exponent
 // per site instead of an FP64 lnScale):
-//   CLV slot   : float  clv[S][C][K]   then   int32 exp[S]
+//   CLV slot   : float  clv[S][C][K]   then   int32 exp[S]   (K != 4)
+//                float  clv[C][S][4]   then   int32 exp[S]   (K == 4: component
+//                planes, so a wave's float4 access covers 1 KiB contiguously)
 //   tip codes  : uint8  codes[ntaxa][S]      index into the tree's code table
 //   P block    : float  P[C][K][K]           row = parent state
 //   tip table  : float  T[ncodes][C][K]      T = P * leaf vector of each code
@@ -37,8 +39,46 @@ struct Op {
     int32_t *parent_exp;
     int32_t nchild;
     int32_t kinds;         // 2 bits per child (ChildKind)
+    int32_t from_stack;    // bit j: child j is read from the lane's LDS stack
+    // Four signed bytes: stack slot of child j in byte j, and in byte 3 the
+    // slot the result is pushed to (-1: not pushed).  Packed in one dword so
+    // the kernel reads it with a scalar load (a byte field would need a vector
+    // load, whose wait also waits for the previous update's stores).
+    int32_t pos;
+    // 4-state kernel: before this update, stage the masks of tip children
+    // [tip_load, tip_load + 64) of the evaluation's tip list into LDS
+    // (-1: none); a tip child's byte in `pos` is then its LDS row.
+    int32_t tip_load;
+    int32_t pad_;
     OpChild ch[kOpMaxChild];
 };
+
+constexpr int kTipChunk = 32;   // tip mask rows staged in LDS at a time
+
+__host__ __device__ inline int op_pos(int32_t packed, int j)
+{
+    return (int)(int8_t)((uint32_t)packed >> (8 * j));
+}
+
+// Sites per block of the 4-state traversal (one wave) and of the generic one.
+constexpr int kTile4 = 64;
+__host__ __device__ inline int tile_sites(int K) { return K == 4 ? kTile4 : kTile; }
+
+// Largest per-lane LDS stack depth of the 4-state traversal for a component
+// class CM (keeps the stack at or below 32 KiB per block).
+__host__ __device__ constexpr int stack_depth4_max(int CM) { return CM <= 4 ? 8 : (CM == 8 ? 4 : 2); }
+
+// Stack depth actually used: the tuned value (0 = largest), rounded down to
+// an instantiated depth (2, 4 or 8).
+__host__ __device__ inline int stack_depth4(int CM, int tuned)
+{
+    // Untuned: 8 slots while a block's LDS stays small enough for every block
+    // of a 60k-site evaluation to be resident at once, else 4.
+    int d = tuned > 0 ? tuned : (CM <= 2 ? 8 : 4);
+    if (d > stack_depth4_max(CM))
+        d = stack_depth4_max(CM);
+    return d >= 8 ? 8 : (d >= 4 ? 4 : 2);
+}
 
 struct Entry {
     const Op *ops;
@@ -53,7 +93,16 @@ struct Entry {
     double *site_out;      // [S] pattern-weighted site lnLs
     double *partials;      // [ntiles][2] compensated block sums
     double *lnl_out;       // final lnL
+    const uint8_t *const *tip_rows;  // 4-state: mask row of each tip child, plan order
+    int32_t ntips;
+    int32_t pad_;
+    // 4-state: P blocks of the plan, dense [op][child][C][16] (child slots
+    // without a P block are left unused), so the kernel can stage them into
+    // LDS a chunk of kPChunk updates at a time.
+    const float *pdense;
 };
+
+constexpr int kPChunk = 8;   // updates whose P blocks are staged in LDS together
 
 struct PJob {
     double len;            // branch length (times comp_rate[c] gives t)

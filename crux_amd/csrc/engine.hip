@@ -13,12 +13,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "crux_lik.h"
@@ -29,7 +31,7 @@ namespace crux {
 
 int traverse_cmax(int K, int C);
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, hipStream_t stream);
+                    int blk0, int nblocks, int depth, hipStream_t stream);
 void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream);
 void launch_finalize(const Entry *entries, int nentries, hipStream_t stream);
 
@@ -67,10 +69,12 @@ struct crux_ctx {
     uint64_t budget = 0;
     uint64_t used = 0;
     int mode = 0;
+    int depth4 = 0;                // 4-state LDS stack depth (0 = largest)
     bool timing = false;
     hipEvent_t ev[4] = {};
     double last_ms[4] = {0, 0, 0, 0};
-    uint64_t last_bytes = 0;
+    uint64_t last_bytes = 0;      // HBM bytes the last evaluation had to move
+    uint64_t last_formula = 0;    // same with every child CLV read from HBM
     int last_nodes = 0;
     // Staging: one pinned host buffer mirrored by one device buffer.
     char *h_plan = nullptr;
@@ -167,6 +171,9 @@ struct PlanChild {
 struct PlanOp {
     int32_t parent_slot;   // -1: root reduce
     std::vector<PlanChild> ch;
+    int32_t from_stack = 0;            // bit j: child j comes from the LDS stack
+    int8_t pos[kOpMaxChild + 1] = {-1, -1, -1, -1};
+    int32_t tip_load = -1;             // first tip index staged before this op
 };
 
 }  // namespace
@@ -176,6 +183,7 @@ struct crux_tree {
     int K = 0, S = 0, ntaxa = 0, ncodes = 0;
     uint8_t *d_codes = nullptr;
     double *d_codevecs = nullptr;
+    uint8_t *d_masks = nullptr;     // 4-state trees: 4-bit state mask per (taxon, site)
     uint32_t *d_weights = nullptr;
     bool has_topology = false;
     int nnodes = 0, nedges = 0;
@@ -490,6 +498,137 @@ static int plan_lik(crux_lik *L, int root)
 // ---------------------------------------------------------------------------
 // Execution of a batch of planned likelihoods.
 
+// Re-sequence a plan for the 4-state kernel's per-lane LDS stack and assign
+// stack slots.  The node updates of a plan form a tree (each update's result
+// is consumed by exactly one later update, the root update by none).  It is
+// re-emitted in post-order with the child subtree that needs the deepest
+// stack first (Sethi-Ullman order), which bounds the depth by about log2 of
+// the number of updates.  Results that would sit deeper than `depth` slots
+// are not pushed; their consumer reads them back from HBM, where every
+// non-root update is stored anyway.
+static void schedule_stack(std::vector<PlanOp> &ops, int depth)
+{
+    const int n = (int)ops.size();
+    if (n == 0)
+        return;
+    // producer[i][j]: op that produced child j of op i in this plan, or -1.
+    std::vector<std::array<int, kOpMaxChild>> producer(n);
+    std::vector<int> consumer(n, -1);
+    {
+        std::unordered_map<int32_t, int> writer;   // slot -> latest op writing it
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < kOpMaxChild; j++)
+                producer[i][j] = -1;
+            for (size_t j = 0; j < ops[i].ch.size(); j++) {
+                const PlanChild &c = ops[i].ch[j];
+                if (c.kind == kChildTip)
+                    continue;
+                auto w = writer.find(c.slot);
+                if (w != writer.end() && consumer[w->second] < 0) {
+                    producer[i][j] = w->second;
+                    consumer[w->second] = i;
+                }
+            }
+            if (ops[i].parent_slot >= 0)
+                writer[ops[i].parent_slot] = i;
+        }
+    }
+    // Stack need per op (children are earlier in the original order).
+    std::vector<int> need(n, 1);
+    std::vector<std::vector<int>> kids(n);
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < kOpMaxChild; j++)
+            if (producer[i][j] >= 0)
+                kids[i].push_back(producer[i][j]);
+        std::stable_sort(kids[i].begin(), kids[i].end(),
+                         [&](int a, int b) { return need[a] > need[b]; });
+        int d = 1;
+        for (size_t k = 0; k < kids[i].size(); k++)
+            d = std::max(d, (int)k + need[kids[i][k]]);
+        need[i] = d;
+    }
+    // Emit post-order from each root (ops without a consumer), deepest first.
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<std::pair<int, size_t>> st;
+    for (int r = 0; r < n; r++) {
+        if (consumer[r] >= 0)
+            continue;
+        st.push_back({r, 0});
+        while (!st.empty()) {
+            auto &top = st.back();
+            if (top.second < kids[top.first].size()) {
+                int k = kids[top.first][top.second++];
+                st.push_back({k, 0});
+            } else {
+                order.push_back(top.first);
+                st.pop_back();
+            }
+        }
+    }
+    // Assign slots by simulating the stack.
+    std::vector<int> push_slot(n, -1);
+    std::vector<PlanOp> out;
+    out.reserve(n);
+    int sp = 0;
+    for (int i : order) {
+        PlanOp op = std::move(ops[i]);
+        op.from_stack = 0;
+        for (int j = 0; j < kOpMaxChild; j++)
+            op.pos[j] = -1;
+        int popped = 0;
+        for (size_t j = 0; j < op.ch.size(); j++) {
+            int p = producer[i][j];
+            if (p >= 0 && push_slot[p] >= 0) {
+                op.from_stack |= 1 << j;
+                op.pos[j] = (int8_t)push_slot[p];
+                popped++;
+            }
+        }
+        sp -= popped;
+        op.pos[kOpMaxChild] = -1;
+        if (consumer[i] >= 0 && sp < depth) {
+            push_slot[i] = sp;
+            op.pos[kOpMaxChild] = (int8_t)sp;
+            sp++;
+        }
+        out.push_back(std::move(op));
+    }
+    ops.swap(out);
+}
+
+// Number the tip children of a 4-state plan in execution order and decide
+// where the kernel stages their state masks into LDS: a chunk of up to
+// kTipChunk rows is loaded before the first update whose tips would not fit
+// in the current chunk (before every update with tips when each update is its
+// own launch).  A tip child's pos byte becomes its row in the chunk.  Returns
+// the taxon row of every tip child in order.
+static std::vector<int32_t> assign_tips(std::vector<PlanOp> &ops, bool per_launch)
+{
+    std::vector<int32_t> taxa;
+    int base = -1;
+    for (PlanOp &op : ops) {
+        op.tip_load = -1;
+        int count = 0;
+        for (const PlanChild &c : op.ch)
+            count += c.kind == kChildTip;
+        if (count == 0)
+            continue;
+        const int k = (int)taxa.size();
+        if (per_launch || base < 0 || k + count - base > kTipChunk) {
+            op.tip_load = k;
+            base = k;
+        }
+        for (size_t j = 0; j < op.ch.size(); j++) {
+            if (op.ch[j].kind != kChildTip)
+                continue;
+            op.pos[j] = (int8_t)((int)taxa.size() - base);
+            taxa.push_back(op.ch[j].taxon);
+        }
+    }
+    return taxa;
+}
+
 static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
 {
     // Classes by (K, CM) for the traversal launches.
@@ -497,6 +636,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         crux_lik *L;
         int idx;    // output index
         int CM;
+        std::vector<int32_t> tips;   // 4-state: taxon row of each tip child
     };
     std::vector<Item> items;
     for (size_t i = 0; i < liks.size(); i++) {
@@ -510,6 +650,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         items.push_back({L, (int)i, cm});
     }
     ctx->last_bytes = 0;
+    ctx->last_formula = 0;
     ctx->last_nodes = 0;
     if (items.empty()) {
         for (size_t i = 0; i < liks.size(); i++)
@@ -524,18 +665,33 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
 
     const bool plain = ctx->mode == 1;
     // Sizes.
-    size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0;
+    size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0, ntips_total = 0;
     size_t shmem = 0;
     for (Item &it : items) {
         crux_lik *L = it.L;
         crux_tree *t = L->tree;
         const int K = t->K, C = L->C;
+        // The 4-state kernel hands results to parents through its LDS stack;
+        // one-launch-per-update mode cannot (LDS does not outlive a launch).
+        if (K == 4 && !plain)
+            schedule_stack(L->ops, stack_depth4(it.CM, ctx->depth4));
+        if (K == 4) {
+            it.tips = assign_tips(L->ops, plain);
+            ntips_total += it.tips.size();
+        }
         nops += L->ops.size();
         if (plain)
             nplain += L->ops.size();
+        if (K == 4)
+            pm_floats += align_up(L->ops.size() * kOpMaxChild * C * 16, 64);
         for (PlanOp &op : L->ops)
             for (PlanChild &c : op.ch) {
-                if (c.kind == kChildInternal) {
+                if (K == 4) {
+                    if (c.kind != kChildIdent) {
+                        njobs++;
+                        shmem = std::max(shmem, (size_t)C * K * sizeof(double));
+                    }
+                } else if (c.kind == kChildInternal) {
                     njobs++;
                     pm_floats += align_up((size_t)C * K * K, 64);
                     shmem = std::max(shmem, (size_t)C * K * sizeof(double));
@@ -545,7 +701,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                     shmem = std::max(shmem, ((size_t)C * K + (size_t)C * K * K) * sizeof(double));
                 }
             }
-        tiles += (t->S + kTile - 1) / kTile;
+        tiles += (t->S + tile_sites(t->K) - 1) / tile_sites(t->K);
     }
     const size_t nent = items.size() + nplain;
     const size_t off_ent = 0;
@@ -555,11 +711,12 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     const size_t nblk_total = plain ? tiles * 0 : tiles;
     size_t blk_count = 0;
     for (Item &it : items) {
-        size_t nt = (it.L->tree->S + kTile - 1) / kTile;
+        size_t nt = (it.L->tree->S + tile_sites(it.L->tree->K) - 1) / tile_sites(it.L->tree->K);
         blk_count += plain ? nt * (it.L->ops.size() + 1) : nt;
     }
     (void)nblk_total;
-    const size_t plan_bytes = align_up(off_blk + blk_count * sizeof(int32_t), 256);
+    const size_t off_tips = align_up(off_blk + blk_count * sizeof(int32_t), 256);
+    const size_t plan_bytes = align_up(off_tips + ntips_total * sizeof(void *), 256);
     if (shmem > 64 * 1024)
         return set_error(CRUX_EINVAL, "P-matrix job too large for LDS");
 
@@ -608,8 +765,8 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     const PJob *djobs = (const PJob *)(D + off_jobs);
     const int32_t *dblk = (const int32_t *)(D + off_blk);
 
-    size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0;
-    uint64_t traffic = 0;
+    size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0, itip = 0;
+    uint64_t traffic = 0, formula = 0;
     int nodes = 0;
     // Fused launches: groups of consecutive items of one (K, CM) class.
     struct Launch {
@@ -624,45 +781,67 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         crux_tree *t = L->tree;
         const int K = t->K, C = L->C;
         const Arena &A = t->arena[L->pol];
-        const size_t ntl = (t->S + kTile - 1) / kTile;
+        const size_t ntl = (t->S + tile_sites(t->K) - 1) / tile_sites(t->K);
         const size_t op0 = io;
         const uint64_t clv_b = (uint64_t)t->S * C * K * 4, exp_b = (uint64_t)t->S * 4;
+        // 4-state: dense [op][child][C][16] P region of this evaluation.
+        const size_t pdense = ipm;
+        if (K == 4)
+            ipm += align_up(L->ops.size() * kOpMaxChild * C * 16, 64);
         for (PlanOp &pop : L->ops) {
+            const size_t olocal = io - op0;
             Op &op = hops[io++];
             memset(&op, 0, sizeof op);
             op.nchild = (int32_t)pop.ch.size();
+            op.from_stack = pop.from_stack;
+            uint32_t packed = 0;
+            for (int j = 0; j <= kOpMaxChild; j++)
+                packed |= (uint32_t)(uint8_t)pop.pos[j] << (8 * j);
+            op.pos = (int32_t)packed;
+            op.tip_load = pop.tip_load;
             if (pop.parent_slot >= 0) {
                 op.parent = A.clv(pop.parent_slot);
                 op.parent_exp = A.exps(pop.parent_slot);
                 traffic += clv_b + exp_b;
+                formula += clv_b + exp_b;
                 nodes++;
             } else {
                 traffic += 4ull * t->S + 8ull * t->S;   // weights in, site lnL out
+                formula += 4ull * t->S + 8ull * t->S;
             }
             for (size_t j = 0; j < pop.ch.size(); j++) {
                 PlanChild &pc = pop.ch[j];
                 OpChild &oc = op.ch[j];
                 op.kinds |= pc.kind << (2 * j);
                 if (pc.kind == kChildTip) {
-                    oc.codes = t->d_codes + (size_t)pc.taxon * t->S;
+                    // 4-state kernel: state masks; otherwise code indices.
+                    oc.codes = (K == 4 ? t->d_masks : t->d_codes) + (size_t)pc.taxon * t->S;
                     traffic += (uint64_t)t->S;
+                    formula += (uint64_t)t->S;
                 } else {
                     oc.clv = A.clv(pc.slot);
                     oc.exps = A.exps(pc.slot);
-                    traffic += clv_b + exp_b;
+                    formula += clv_b + exp_b;
+                    if (!((pop.from_stack >> j) & 1))
+                        traffic += clv_b + exp_b;
                 }
                 if (pc.kind != kChildIdent) {
                     PJob &jb = hjobs[ij++];
                     jb.len = pc.len;
                     jb.model = L->d_model;
                     jb.codevecs = t->d_codevecs;
-                    jb.out = ctx->d_pmat + ipm;
-                    jb.tip = pc.kind == kChildTip;
+                    // The 4-state kernel forms tip products from P itself.
+                    jb.tip = pc.kind == kChildTip && K != 4;
                     jb.C = C;
                     jb.K = K;
                     jb.ncodes = t->ncodes;
+                    if (K == 4) {
+                        jb.out = ctx->d_pmat + pdense + (olocal * kOpMaxChild + j) * C * 16;
+                    } else {
+                        jb.out = ctx->d_pmat + ipm;
+                        ipm += align_up(jb.tip ? (size_t)t->ncodes * C * K : (size_t)C * K * K, 64);
+                    }
                     oc.mat = jb.out;
-                    ipm += align_up(jb.tip ? (size_t)t->ncodes * C * K : (size_t)C * K * K, 64);
                 }
             }
         }
@@ -678,6 +857,16 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         base.site_out = L->d_site;
         base.partials = ctx->d_partials + 2 * itile;
         base.lnl_out = ctx->d_lnl + items[ii].idx;
+        base.ntips = (int32_t)items[ii].tips.size();
+        base.tip_rows = nullptr;
+        base.pdense = K == 4 ? ctx->d_pmat + pdense : nullptr;
+        if (!items[ii].tips.empty()) {
+            const uint8_t **hrows = (const uint8_t **)(H + off_tips) + itip;
+            for (size_t k = 0; k < items[ii].tips.size(); k++)
+                hrows[k] = t->d_masks + (size_t)items[ii].tips[k] * t->S;
+            base.tip_rows = (const uint8_t *const *)(D + off_tips) + itip;
+            itip += items[ii].tips.size();
+        }
         itile += ntl;
         if (!plain) {
             base.tile0 = (int32_t)iblk;
@@ -696,6 +885,8 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                 Entry e1 = base;
                 e1.ops = dops + op0 + o;
                 e1.nops = 1;
+                if (base.pdense)
+                    e1.pdense = base.pdense + o * kOpMaxChild * C * 16;
                 e1.tile0 = (int32_t)iblk;
                 hent[ient] = e1;
                 for (size_t b = 0; b < ntl; b++)
@@ -719,7 +910,8 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     for (Launch &ln : launches) {
         if (ln.nblk == 0)
             continue;
-        if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk, st) != 0)
+        if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
+                            stack_depth4(ln.CM, ctx->depth4), st) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
         HIP_TRY(hipGetLastError());
     }
@@ -755,6 +947,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         ctx->last_ms[3] = d;
     }
     ctx->last_bytes = traffic;
+    ctx->last_formula = formula;
     ctx->last_nodes = nodes;
     for (Item &it : items) {
         it.L->last_lnl = ctx->h_lnl[it.idx];
@@ -843,6 +1036,34 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode)
     return CRUX_OK;
 }
 
+int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value)
+{
+    if (!ctx)
+        return set_error(CRUX_EINVAL, "ctx is NULL");
+    switch (key) {
+    case CRUX_TUNE_STACK_DEPTH:
+        if (value < 0 || value > 64)
+            return set_error(CRUX_EINVAL, "stack depth %d", value);
+        ctx->depth4 = value;
+        return CRUX_OK;
+    default:
+        return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
+    }
+}
+
+int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value)
+{
+    if (!ctx || !value)
+        return set_error(CRUX_EINVAL, "NULL argument");
+    switch (key) {
+    case CRUX_TUNE_STACK_DEPTH:
+        *value = ctx->depth4;
+        return CRUX_OK;
+    default:
+        return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
+    }
+}
+
 int crux_ctx_set_timing(crux_ctx *ctx, int on)
 {
     if (!ctx)
@@ -860,12 +1081,15 @@ int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4)
     return CRUX_OK;
 }
 
-int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, int *nodes)
+int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, uint64_t *formula_bytes,
+                          int *nodes)
 {
     if (!ctx)
         return set_error(CRUX_EINVAL, "ctx is NULL");
     if (bytes)
         *bytes = ctx->last_bytes;
+    if (formula_bytes)
+        *formula_bytes = ctx->last_formula;
     if (nodes)
         *nodes = ctx->last_nodes;
     return CRUX_OK;
@@ -904,6 +1128,25 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
                       hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(t->d_weights, weights, sizeof(uint32_t) * nsites, hipMemcpyHostToDevice);
+    if (e == hipSuccess && dim == 4) {
+        // The 4-state kernel reads the leaf vector itself as a 4-bit mask.
+        std::vector<uint8_t> mask_of(ncodes), masks((size_t)ntaxa * nsites);
+        for (int c = 0; c < ncodes; c++) {
+            uint8_t m = 0;
+            for (int j = 0; j < 4; j++)
+                if (code_vecs[(size_t)c * 4 + j] != 0.0)
+                    m |= (uint8_t)(1u << j);
+            mask_of[c] = m;
+        }
+        for (size_t i = 0; i < masks.size(); i++)
+            masks[i] = mask_of[codes[i]];
+        rc = dev_alloc(ctx, (void **)&t->d_masks, masks.size());
+        if (rc) {
+            crux_tree_destroy(t);
+            return rc;
+        }
+        e = hipMemcpy(t->d_masks, masks.data(), masks.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         crux_tree_destroy(t);
         return set_error(CRUX_EDEVICE, "tree upload: %s", hipGetErrorString(e));
@@ -922,6 +1165,7 @@ int crux_tree_destroy(crux_tree *t)
     arena_reset(t, 1);
     dev_free(ctx, t->d_codes, (size_t)t->ntaxa * t->S);
     dev_free(ctx, t->d_codevecs, sizeof(double) * t->ncodes * t->K);
+    dev_free(ctx, t->d_masks, (size_t)t->ntaxa * t->S);
     dev_free(ctx, t->d_weights, sizeof(uint32_t) * t->S);
     for (int p = 0; p < 2; p++)
         if (t->owner[p])

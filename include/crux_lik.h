@@ -158,16 +158,29 @@ int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks,
  *           the "one call per eval, no fusion" comparison path. */
 int crux_ctx_set_mode(crux_ctx *ctx, int mode);
 
+/* Launch-shape tuning of the context (set by the autotuner,
+ * crux_amd/autotune.py):
+ *   CRUX_TUNE_STACK_DEPTH: LDS stack slots per lane of the 4-state kernel
+ *   (2, 4 or 8; 0 = the largest that fits).  Fewer slots let more blocks
+ *   share a CU; more slots keep more pending results out of HBM. */
+enum { CRUX_TUNE_STACK_DEPTH = 1 };
+int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
+int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
+
 /* Timing of the last evaluation on the device stream, in milliseconds:
  * [0] P-matrix kernel, [1] traversal kernel(s), [2] reduction, [3] total.
  * Only filled when timing was enabled with crux_ctx_set_timing(ctx, 1). */
 int crux_ctx_set_timing(crux_ctx *ctx, int on);
 int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4);
 
-/* Per-call algorithmic traffic of the last evaluation in bytes (CLV reads and
- * writes, tip codes, scale exponents, weights) and the number of node
- * updates it ran. */
-int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, int *nodes);
+/* Algorithmic HBM traffic of the last evaluation in bytes: `bytes` counts
+ * what the kernels must move (CLV and exponent writes, CLVs of children cached
+ * by earlier evaluations, tip codes, weights, site lnLs); `formula_bytes`
+ * counts every child CLV as read from HBM (the per-node-update byte model
+ * B = sum(child reads) + parent write).  `nodes` is the number of node
+ * updates. */
+int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, uint64_t *formula_bytes,
+                          int *nodes);
 
 #ifdef __cplusplus
 }
