@@ -100,7 +100,13 @@ struct Entry {
     // without a P block are left unused), so the kernel can stage them into
     // LDS a chunk of kPChunk updates at a time.
     const float *pdense;
+    // 4-state: control words of each update, [op][8]: nchild | kinds << 8 |
+    // from_stack << 16, pos, tip_load, has_parent, parent (2 words),
+    // parent_exp (2 words); staged into LDS with the P chunks.
+    const int32_t *ctl;
 };
+
+constexpr int kCtlWords = 8;
 
 constexpr int kPChunk = 8;   // updates whose P blocks are staged in LDS together
 

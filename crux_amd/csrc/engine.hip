@@ -716,7 +716,8 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     }
     (void)nblk_total;
     const size_t off_tips = align_up(off_blk + blk_count * sizeof(int32_t), 256);
-    const size_t plan_bytes = align_up(off_tips + ntips_total * sizeof(void *), 256);
+    const size_t off_ctl = align_up(off_tips + ntips_total * sizeof(void *), 256);
+    const size_t plan_bytes = align_up(off_ctl + nops * kCtlWords * sizeof(int32_t), 256);
     if (shmem > 64 * 1024)
         return set_error(CRUX_EINVAL, "P-matrix job too large for LDS");
 
@@ -799,6 +800,19 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                 packed |= (uint32_t)(uint8_t)pop.pos[j] << (8 * j);
             op.pos = (int32_t)packed;
             op.tip_load = pop.tip_load;
+            auto write_ctl = [&]() {
+                int32_t *cw = (int32_t *)(H + off_ctl) + (io - 1) * kCtlWords;
+                cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16);
+                cw[1] = op.pos;
+                cw[2] = op.tip_load;
+                cw[3] = pop.parent_slot >= 0 ? 1 : 0;
+                const uint64_t pp = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.clv(pop.parent_slot) : nullptr);
+                const uint64_t pe = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.exps(pop.parent_slot) : nullptr);
+                cw[4] = (int32_t)(uint32_t)pp;
+                cw[5] = (int32_t)(uint32_t)(pp >> 32);
+                cw[6] = (int32_t)(uint32_t)pe;
+                cw[7] = (int32_t)(uint32_t)(pe >> 32);
+            };
             if (pop.parent_slot >= 0) {
                 op.parent = A.clv(pop.parent_slot);
                 op.parent_exp = A.exps(pop.parent_slot);
@@ -844,9 +858,11 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                     oc.mat = jb.out;
                 }
             }
+            write_ctl();
         }
         Entry base{};
         base.ops = dops + op0;
+        base.ctl = (const int32_t *)(D + off_ctl) + op0 * kCtlWords;
         base.nops = (int32_t)L->ops.size();
         base.S = t->S;
         base.C = C;
@@ -887,6 +903,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                 e1.nops = 1;
                 if (base.pdense)
                     e1.pdense = base.pdense + o * kOpMaxChild * C * 16;
+                e1.ctl = base.ctl + o * kCtlWords;
                 e1.tile0 = (int32_t)iblk;
                 hent[ient] = e1;
                 for (size_t b = 0; b < ntl; b++)

@@ -435,13 +435,19 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
     const int chunk_f = kPChunk * kOpMaxChild * C * 16;
     const int total_f = nops * kOpMaxChild * C * 16;
     const CRUX_GAS float *pd = gas(E->pdense);
+    // The updates' control words travel with the P chunks for the same reason.
+    __shared__ int32_t cch[2][kPChunk * kCtlWords];
+    const CRUX_GAS int32_t *cg = gas(E->ctl);
     float pf[kPer];
+    int32_t cf = 0;
     auto fetch = [&](int k) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int i = tid * kPer + q, g = k * chunk_f + i;
             pf[q] = (i < chunk_f && g < total_f) ? pd[g] : 0.0f;
         }
+        if (tid < kPChunk * kCtlWords)
+            cf = (k * kPChunk + tid / kCtlWords < nops) ? cg[k * kPChunk * kCtlWords + tid] : 0;
     };
     auto commit = [&](int buf) {
 #pragma unroll
@@ -450,6 +456,8 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
             if (i < chunk_f)
                 pch[buf][i] = pf[q];
         }
+        if (tid < kPChunk * kCtlWords)
+            cch[buf][tid] = cf;
     };
     fetch(0);
     commit(0);
@@ -464,9 +472,14 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
             __syncthreads();
             fetch(k + 1);
         }
-        const float *pchunk = &pch[(o / kPChunk) & 1][(o % kPChunk) * kOpMaxChild * C * 16];
-
-        const int tl = op->tip_load;
+        const int buf = (o / kPChunk) & 1;
+        const float *pchunk = &pch[buf][(o % kPChunk) * kOpMaxChild * C * 16];
+        const int32_t *cw = &cch[buf][(o % kPChunk) * kCtlWords];
+        const int w0 = __builtin_amdgcn_readfirstlane(cw[0]);
+        const int32_t pk = __builtin_amdgcn_readfirstlane(cw[1]);
+        const int tl = __builtin_amdgcn_readfirstlane(cw[2]);
+        const bool has_parent = __builtin_amdgcn_readfirstlane(cw[3]) != 0;
+        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xff;
         if (tl >= 0) {
             // Stage the next chunk of tip state masks; waves split the rows.
             __syncthreads();
@@ -479,8 +492,6 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
             __syncthreads();
         }
 
-        const int nch = op->nchild, kinds = op->kinds, fst = op->from_stack;
-        const int32_t pk = op->pos;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
         int32_t ex = 0;
         if (wc) {
@@ -517,7 +528,7 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                 acc = (j == 0) ? v : mul4(acc, v);
             }
         }
-        if (op->parent != nullptr) {
+        if (has_parent) {
             smax[o & 1][c][lane] = wc ? max4(acc) : 0.0f;
             __syncthreads();
             float m = smax[o & 1][0][lane];
@@ -532,14 +543,19 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                 const float4 y = make_float4(ldexpf(acc.x, -e2), ldexpf(acc.y, -e2),
                                              ldexpf(acc.z, -e2), ldexpf(acc.w, -e2));
                 const int pp = op_pos(pk, kOpMaxChild);
+                const uint64_t par = (uint32_t)__builtin_amdgcn_readfirstlane(cw[4]) |
+                                     ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[5]) << 32);
                 if (act)
-                    st4(gas(reinterpret_cast<float4 *>(op->parent)) + (size_t)c * S + s, y);
+                    st4(gas(reinterpret_cast<float4 *>(par)) + (size_t)c * S + s, y);
                 if (pp >= 0) {
                     stk[pp][c][lane] = y;
                     stke[pp][c][lane] = ex + e2;
                 }
-                if (c == 0 && act)
-                    gas(op->parent_exp)[s] = ex + e2;
+                if (c == 0 && act) {
+                    const uint64_t pe = (uint32_t)__builtin_amdgcn_readfirstlane(cw[6]) |
+                                        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[7]) << 32);
+                    gas(reinterpret_cast<int32_t *>(pe))[s] = ex + e2;
+                }
             }
         } else {
             double Lc = 0.0;
