@@ -33,7 +33,11 @@ def build(verbose=False):
         obj = os.path.join(CSRC, "_" + src + ".o")
         if src.endswith(".hip"):
             cmd = [hipcc(), "--offload-arch=" + ARCH, "-x", "hip", "-Xclang",
-                   "-target-feature", "-Xclang", "-packed-fp32-ops", "-c",
+                   "-target-feature", "-Xclang", "-packed-fp32-ops",
+                   # Matrix-core results straight into VGPRs: the 4-state
+                   # kernel feeds them to vector instructions at once, and
+                   # accumulation registers would cost a move per value.
+                   "-mllvm", "-amdgpu-mfma-vgpr-form", "-c",
                    path, "-o", obj] + common
         else:
             cmd = [hipcc(), "-c", path, "-o", obj] + common

@@ -7,6 +7,8 @@
 //                float  clv[C][S][4]   then   int32 exp[S]   (K == 4: component
 //                planes, so a wave's float4 access covers 1 KiB contiguously)
 //   tip codes  : uint8  codes[ntaxa][S]      index into the tree's code table
+//   tip masks  : uint8  masks[ntaxa][S64]    K == 4: 4-bit state set, rows
+//                padded to S64 = S rounded up to 64
 //   P block    : float  P[C][K][K]           row = parent state
 //   tip table  : float  T[ncodes][C][K]      T = P * leaf vector of each code
 //   root weight: double piw[C][K]            pi_norm * weightScaled
@@ -45,15 +47,16 @@ struct Op {
     // the kernel reads it with a scalar load (a byte field would need a vector
     // load, whose wait also waits for the previous update's stores).
     int32_t pos;
-    // 4-state kernel: before this update, stage the masks of tip children
-    // [tip_load, tip_load + 64) of the evaluation's tip list into LDS
-    // (-1: none); a tip child's byte in `pos` is then its LDS row.
+    // 4-state kernel: before this update, stage the state masks of tip
+    // children [tip_load, tip_load + kTipChunk) of the entry's tip list into
+    // LDS (-1: none; the chunk at 0 is staged when the launch starts); a tip
+    // child's byte in `pos` is its row in the staged chunk.
     int32_t tip_load;
     int32_t pad_;
     OpChild ch[kOpMaxChild];
 };
 
-constexpr int kTipChunk = 32;   // tip mask rows staged in LDS at a time
+constexpr int kTipChunk = 64;   // tip mask rows staged in LDS at a time (power of 2)
 
 __host__ __device__ inline int op_pos(int32_t packed, int j)
 {
@@ -96,19 +99,20 @@ struct Entry {
     const uint8_t *const *tip_rows;  // 4-state: mask row of each tip child, plan order
     int32_t ntips;
     int32_t pad_;
-    // 4-state: P blocks of the plan, dense [op][child][C][16] (child slots
-    // without a P block are left unused), so the kernel can stage them into
-    // LDS a chunk of kPChunk updates at a time.
+    // 4-state: P blocks of the plan, dense [op][child][CM][16] with CM the
+    // component class (rows of components >= C zero; child slots without a P
+    // block are left unused), staged into LDS a chunk of updates at a time.
     const float *pdense;
     // 4-state: control words of each update, [op][8]: nchild | kinds << 8 |
-    // from_stack << 16, pos, tip_load, has_parent, parent (2 words),
-    // parent_exp (2 words); staged into LDS with the P chunks.
+    // from_stack << 16 | forwarded << 20 (bit j: child j is the previous
+    // update's result, kept in registers), pos, tip_load, has_parent, parent
+    // (2 words), parent_exp (2 words); staged with the P blocks.  The tip
+    // list (tip_rows, ntips) is per entry, so a one-update entry starts its
+    // own.
     const int32_t *ctl;
 };
 
 constexpr int kCtlWords = 8;
-
-constexpr int kPChunk = 8;   // updates whose P blocks are staged in LDS together
 
 struct PJob {
     double len;            // branch length (times comp_rate[c] gives t)
@@ -119,6 +123,8 @@ struct PJob {
     int32_t C;
     int32_t K;
     int32_t ncodes;
+    int32_t cm;            // P blocks: rows written up to cm * K * K (zero past C)
+    int32_t pad_;
 };
 
 __host__ __device__ inline int model_stride(int K) { return 2 * K * K + K + 1; }

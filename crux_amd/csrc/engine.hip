@@ -143,7 +143,7 @@ constexpr int32_t kRootKey = -1;
 
 struct Arena {
     int C = 0;
-    size_t slot_bytes = 0;   // CLV [S][C][K] floats + exps [S] int32, 256-aligned
+    size_t slot_bytes = 0;   // CLV floats + int32 exponents, each 256-aligned
     size_t clv_bytes = 0;
     int chunk_slots = 0;
     std::vector<char *> chunks;
@@ -172,8 +172,11 @@ struct PlanOp {
     int32_t parent_slot;   // -1: root reduce
     std::vector<PlanChild> ch;
     int32_t from_stack = 0;            // bit j: child j comes from the LDS stack
+    int32_t fwd = 0;                   // bit j: child j is the previous update's result
     int8_t pos[kOpMaxChild + 1] = {-1, -1, -1, -1};
-    int32_t tip_load = -1;             // first tip index staged before this op
+    int32_t tip_load = -1;             // tip chunk staged before this op (entry-relative)
+    int32_t tip_first = 0;             // index of the op's first tip in the evaluation's list
+    int32_t tip_count = 0;
 };
 
 }  // namespace
@@ -192,6 +195,10 @@ struct crux_tree {
     std::vector<RingCache> cache[2];
     Arena arena[2];
     crux_lik *owner[2] = {nullptr, nullptr};
+
+    size_t mask_stride() const { return align_up((size_t)S, kTile4); }
+    // One power-of-two exponent per site, shared by all components.
+    size_t exp_count(int C) const { (void)C; return (size_t)S; }
 };
 
 struct crux_lik {
@@ -264,7 +271,7 @@ static int arena_layout(crux_tree *t, int pol, int C)
     arena_reset(t, pol);
     a.C = C;
     a.clv_bytes = align_up((size_t)t->S * C * t->K * sizeof(float), 256);
-    a.slot_bytes = a.clv_bytes + align_up((size_t)t->S * sizeof(int32_t), 256);
+    a.slot_bytes = a.clv_bytes + align_up(t->exp_count(C) * sizeof(int32_t), 256);
     a.chunk_slots = 0;
     return CRUX_OK;
 }
@@ -566,20 +573,28 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
             }
         }
     }
-    // Assign slots by simulating the stack.
+    // Assign slots by simulating the stack.  A result whose consumer is the
+    // very next update stays in the kernel's registers (forwarded) and takes
+    // no slot; in post-order that is the last child of every update with an
+    // internal child, so the stack only holds the earlier siblings.
     std::vector<int> push_slot(n, -1);
+    std::vector<char> forwarded(n, 0);
     std::vector<PlanOp> out;
     out.reserve(n);
     int sp = 0;
-    for (int i : order) {
+    for (size_t k = 0; k < order.size(); k++) {
+        const int i = order[k];
         PlanOp op = std::move(ops[i]);
         op.from_stack = 0;
+        op.fwd = 0;
         for (int j = 0; j < kOpMaxChild; j++)
             op.pos[j] = -1;
         int popped = 0;
         for (size_t j = 0; j < op.ch.size(); j++) {
             int p = producer[i][j];
-            if (p >= 0 && push_slot[p] >= 0) {
+            if (p >= 0 && forwarded[p]) {
+                op.fwd |= 1 << j;
+            } else if (p >= 0 && push_slot[p] >= 0) {
                 op.from_stack |= 1 << j;
                 op.pos[j] = (int8_t)push_slot[p];
                 popped++;
@@ -587,7 +602,9 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
         }
         sp -= popped;
         op.pos[kOpMaxChild] = -1;
-        if (consumer[i] >= 0 && sp < depth) {
+        if (consumer[i] >= 0 && k + 1 < order.size() && order[k + 1] == consumer[i]) {
+            forwarded[i] = 1;
+        } else if (consumer[i] >= 0 && sp < depth) {
             push_slot[i] = sp;
             op.pos[kOpMaxChild] = (int8_t)sp;
             sp++;
@@ -598,31 +615,34 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
 }
 
 // Number the tip children of a 4-state plan in execution order and decide
-// where the kernel stages their state masks into LDS: a chunk of up to
-// kTipChunk rows is loaded before the first update whose tips would not fit
-// in the current chunk (before every update with tips when each update is its
-// own launch).  A tip child's pos byte becomes its row in the chunk.  Returns
-// the taxon row of every tip child in order.
+// where the kernel stages their state masks into LDS: the chunk of up to
+// kTipChunk rows starting at tip 0 is staged when the launch starts, and a
+// new chunk starts at the first tip of the first update whose tips would not
+// fit in the current one.  A tip child's pos byte is its row in its chunk.
+// When each update is its own launch, the update's tips form the launch's own
+// list (tip_first, tip_count), staged at its start.  Returns the taxon row of
+// every tip child in order.
 static std::vector<int32_t> assign_tips(std::vector<PlanOp> &ops, bool per_launch)
 {
     std::vector<int32_t> taxa;
-    int base = -1;
+    int base = 0;
     for (PlanOp &op : ops) {
         op.tip_load = -1;
-        int count = 0;
+        op.tip_first = (int32_t)taxa.size();
+        op.tip_count = 0;
         for (const PlanChild &c : op.ch)
-            count += c.kind == kChildTip;
-        if (count == 0)
+            op.tip_count += c.kind == kChildTip;
+        if (op.tip_count == 0)
             continue;
         const int k = (int)taxa.size();
-        if (per_launch || base < 0 || k + count - base > kTipChunk) {
+        if (!per_launch && k + op.tip_count - base > kTipChunk) {
             op.tip_load = k;
             base = k;
         }
         for (size_t j = 0; j < op.ch.size(); j++) {
             if (op.ch[j].kind != kChildTip)
                 continue;
-            op.pos[j] = (int8_t)((int)taxa.size() - base);
+            op.pos[j] = (int8_t)(per_launch ? (int)taxa.size() - k : (int)taxa.size() - base);
             taxa.push_back(op.ch[j].taxon);
         }
     }
@@ -683,7 +703,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         if (plain)
             nplain += L->ops.size();
         if (K == 4)
-            pm_floats += align_up(L->ops.size() * kOpMaxChild * C * 16, 64);
+            pm_floats += align_up(L->ops.size() * kOpMaxChild * it.CM * 16, 64);
         for (PlanOp &op : L->ops)
             for (PlanChild &c : op.ch) {
                 if (K == 4) {
@@ -784,11 +804,12 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         const Arena &A = t->arena[L->pol];
         const size_t ntl = (t->S + tile_sites(t->K) - 1) / tile_sites(t->K);
         const size_t op0 = io;
-        const uint64_t clv_b = (uint64_t)t->S * C * K * 4, exp_b = (uint64_t)t->S * 4;
-        // 4-state: dense [op][child][C][16] P region of this evaluation.
+        const uint64_t clv_b = (uint64_t)t->S * C * K * 4, exp_b = (uint64_t)t->exp_count(C) * 4;
+        // 4-state: dense [op][child][CM][16] P region of this evaluation.
         const size_t pdense = ipm;
+        const int CM = items[ii].CM;
         if (K == 4)
-            ipm += align_up(L->ops.size() * kOpMaxChild * C * 16, 64);
+            ipm += align_up(L->ops.size() * kOpMaxChild * CM * 16, 64);
         for (PlanOp &pop : L->ops) {
             const size_t olocal = io - op0;
             Op &op = hops[io++];
@@ -802,7 +823,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
             op.tip_load = pop.tip_load;
             auto write_ctl = [&]() {
                 int32_t *cw = (int32_t *)(H + off_ctl) + (io - 1) * kCtlWords;
-                cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16);
+                cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20);
                 cw[1] = op.pos;
                 cw[2] = op.tip_load;
                 cw[3] = pop.parent_slot >= 0 ? 1 : 0;
@@ -829,14 +850,15 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                 op.kinds |= pc.kind << (2 * j);
                 if (pc.kind == kChildTip) {
                     // 4-state kernel: state masks; otherwise code indices.
-                    oc.codes = (K == 4 ? t->d_masks : t->d_codes) + (size_t)pc.taxon * t->S;
+                    oc.codes = K == 4 ? t->d_masks + (size_t)pc.taxon * t->mask_stride()
+                                      : t->d_codes + (size_t)pc.taxon * t->S;
                     traffic += (uint64_t)t->S;
                     formula += (uint64_t)t->S;
                 } else {
                     oc.clv = A.clv(pc.slot);
                     oc.exps = A.exps(pc.slot);
                     formula += clv_b + exp_b;
-                    if (!((pop.from_stack >> j) & 1))
+                    if (!(((pop.from_stack | pop.fwd) >> j) & 1))
                         traffic += clv_b + exp_b;
                 }
                 if (pc.kind != kChildIdent) {
@@ -849,8 +871,9 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                     jb.C = C;
                     jb.K = K;
                     jb.ncodes = t->ncodes;
+                    jb.cm = K == 4 && !jb.tip ? CM : C;
                     if (K == 4) {
-                        jb.out = ctx->d_pmat + pdense + (olocal * kOpMaxChild + j) * C * 16;
+                        jb.out = ctx->d_pmat + pdense + (olocal * kOpMaxChild + j) * CM * 16;
                     } else {
                         jb.out = ctx->d_pmat + ipm;
                         ipm += align_up(jb.tip ? (size_t)t->ncodes * C * K : (size_t)C * K * K, 64);
@@ -879,7 +902,7 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         if (!items[ii].tips.empty()) {
             const uint8_t **hrows = (const uint8_t **)(H + off_tips) + itip;
             for (size_t k = 0; k < items[ii].tips.size(); k++)
-                hrows[k] = t->d_masks + (size_t)items[ii].tips[k] * t->S;
+                hrows[k] = t->d_masks + (size_t)items[ii].tips[k] * t->mask_stride();
             base.tip_rows = (const uint8_t *const *)(D + off_tips) + itip;
             itip += items[ii].tips.size();
         }
@@ -902,8 +925,12 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                 e1.ops = dops + op0 + o;
                 e1.nops = 1;
                 if (base.pdense)
-                    e1.pdense = base.pdense + o * kOpMaxChild * C * 16;
+                    e1.pdense = base.pdense + o * kOpMaxChild * CM * 16;
                 e1.ctl = base.ctl + o * kCtlWords;
+                if (base.tip_rows) {
+                    e1.tip_rows = base.tip_rows + L->ops[o].tip_first;
+                    e1.ntips = L->ops[o].tip_count;
+                }
                 e1.tile0 = (int32_t)iblk;
                 hent[ient] = e1;
                 for (size_t b = 0; b < ntl; b++)
@@ -1146,8 +1173,9 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
     if (e == hipSuccess)
         e = hipMemcpy(t->d_weights, weights, sizeof(uint32_t) * nsites, hipMemcpyHostToDevice);
     if (e == hipSuccess && dim == 4) {
-        // The 4-state kernel reads the leaf vector itself as a 4-bit mask.
-        std::vector<uint8_t> mask_of(ncodes), masks((size_t)ntaxa * nsites);
+        // The 4-state kernel reads the leaf vector itself as a 4-bit mask;
+        // rows are padded to whole 64-site tiles (read four sites per lane).
+        std::vector<uint8_t> mask_of(ncodes), masks((size_t)ntaxa * t->mask_stride(), 0);
         for (int c = 0; c < ncodes; c++) {
             uint8_t m = 0;
             for (int j = 0; j < 4; j++)
@@ -1155,8 +1183,9 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
                     m |= (uint8_t)(1u << j);
             mask_of[c] = m;
         }
-        for (size_t i = 0; i < masks.size(); i++)
-            masks[i] = mask_of[codes[i]];
+        for (int x = 0; x < ntaxa; x++)
+            for (int s = 0; s < nsites; s++)
+                masks[(size_t)x * t->mask_stride() + s] = mask_of[codes[(size_t)x * nsites + s]];
         rc = dev_alloc(ctx, (void **)&t->d_masks, masks.size());
         if (rc) {
             crux_tree_destroy(t);
@@ -1182,7 +1211,7 @@ int crux_tree_destroy(crux_tree *t)
     arena_reset(t, 1);
     dev_free(ctx, t->d_codes, (size_t)t->ntaxa * t->S);
     dev_free(ctx, t->d_codevecs, sizeof(double) * t->ncodes * t->K);
-    dev_free(ctx, t->d_masks, (size_t)t->ntaxa * t->S);
+    dev_free(ctx, t->d_masks, (size_t)t->ntaxa * t->mask_stride());
     dev_free(ctx, t->d_weights, sizeof(uint32_t) * t->S);
     for (int p = 0; p < 2; p++)
         if (t->owner[p])

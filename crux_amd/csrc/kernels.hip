@@ -133,6 +133,10 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
         else
             J.out[idx] = (float)p;
     }
+    // 4-state P blocks are strided by the component class: the rows of the
+    // idle components c in [C, cm) are zero.
+    for (int idx = C * KK + (int)threadIdx.x; idx < J.cm * KK; idx += blockDim.x)
+        J.out[idx] = 0.0f;
     if (!J.tip)
         return;
     __syncthreads();
@@ -331,260 +335,370 @@ k_traverse(const Entry *__restrict__ entries, const int32_t *__restrict__ block_
 }
 
 // ---------------------------------------------------------------------------
-// 4-state traversal.  One lane owns one site across all components; a block is
-// one wave of 64 sites.  A node update computed earlier in the same plan is
-// handed to its parent through a per-lane stack in LDS (post-order makes the
-// pending results LIFO), so a full traversal reads no CLV back from HBM: it
-// only streams the cache writes.  Children cached by an earlier plan come
-// from HBM.  A tip contributes P * leaf vector, formed in registers from the
-// 4-bit state mask, so tips cost one byte per site.  The 16 P entries of a
-// component are uniform across the block and live in scalar registers.
-
-__device__ __forceinline__ float4 pmul4(const CRUX_CAS float *__restrict__ P, float4 x)
-{
-    float4 v;
-    v.x = fmaf(P[0], x.x, fmaf(P[1], x.y, fmaf(P[2], x.z, P[3] * x.w)));
-    v.y = fmaf(P[4], x.x, fmaf(P[5], x.y, fmaf(P[6], x.z, P[7] * x.w)));
-    v.z = fmaf(P[8], x.x, fmaf(P[9], x.y, fmaf(P[10], x.z, P[11] * x.w)));
-    v.w = fmaf(P[12], x.x, fmaf(P[13], x.y, fmaf(P[14], x.z, P[15] * x.w)));
-    return v;
-}
+// 4-state helpers.
 
 __device__ __forceinline__ float4 mul4(float4 a, float4 b)
 {
     return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
 }
 
-__device__ __forceinline__ float max4(float4 a)
+// y = P x for the wave's 64 sites on the matrix core, FP32 in and out:
+// v_mfma_f32_4x4x1f32 computes 16 independent 4x4 outer products, block b
+// taking the A value of lanes 4b..4b+3 as a column and the B value of the
+// same lanes as a row.  With lane l supplying A = P[l % 4][k] and B = its own
+// x[k], block b accumulates y[i] of site 4b+j in result register i of lane
+// 4b+j, so after k = 0..3 every lane holds P x of its own site.  The sum is
+// the same FMA chain P[i][0] x0 + ... + P[i][3] x3 as on the vector ALU, at
+// twice the FP32 rate and off the vector pipeline.
+__device__ __forceinline__ float4 pmul4_mfma(float4 prow, float4 x)
 {
-    return fmaxf(fmaxf(a.x, a.y), fmaxf(a.z, a.w));
-}
-
-// P block of component c for every child of an update, in scalar registers.
-__device__ __forceinline__ void load_p4(const CRUX_CAS Op *op, int c, float (&p)[kOpMaxChild][16])
-{
-    const int nch = op->nchild, kinds = op->kinds;
-#pragma unroll
-    for (int j = 0; j < kOpMaxChild; ++j) {
-        if (j < nch && ((kinds >> (2 * j)) & 3) != kChildIdent) {
-            const CRUX_CAS float *P = cas(op->ch[j].mat) + 16 * c;
-#pragma unroll
-            for (int k = 0; k < 16; ++k)
-                p[j][k] = P[k];
-        }
-    }
-}
-
-__device__ __forceinline__ float4 pmul4r(const float (&P)[16], float4 x)
-{
-    float4 v;
-    v.x = fmaf(P[0], x.x, fmaf(P[1], x.y, fmaf(P[2], x.z, P[3] * x.w)));
-    v.y = fmaf(P[4], x.x, fmaf(P[5], x.y, fmaf(P[6], x.z, P[7] * x.w)));
-    v.z = fmaf(P[8], x.x, fmaf(P[9], x.y, fmaf(P[10], x.z, P[11] * x.w)));
-    v.w = fmaf(P[12], x.x, fmaf(P[13], x.y, fmaf(P[14], x.z, P[15] * x.w)));
-    return v;
+    crux_f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.x, x.x, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.y, x.y, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.z, x.z, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.w, x.w, d, 0, 0, 0);
+    return make_float4(d.x, d.y, d.z, d.w);
 }
 
 // ---------------------------------------------------------------------------
-// 4-state traversal.  A block covers 64 sites with one wave per mixture
-// component (CM waves; waves c >= C idle), so the 16 P entries a wave needs
-// are uniform and sit in scalar registers, and each wave does one
-// component's share of the arithmetic.  The block walks the whole plan
-// (post-order list of fused node updates) for its sites: one launch covers a
-// full traversal and every evaluation of a batch, with no dependency between
-// blocks.  Per update: each wave forms P_c * child_c for every child (a tip
-// contributes P_c * its 4-bit state mask, one byte per site) and multiplies
-// them; the per-site maximum over components is exchanged through LDS (one
-// barrier), giving the exponent e with max in [0.5, 1) after an exact
-// scaling by 2^-e.  The scaled component plane is stored to the HBM cache
-// slot (layout [C][S][4], 1 KiB contiguous per wave store) and, when the
-// parent is in the same plan, pushed onto the wave's LDS stack; parents pop
-// children from there, so a full traversal reads no CLV back from HBM.  The
-// root update reduces to the pattern-weighted FP64 site lnL (NaN -> -inf,
-// CxLik.c:598-632) and a compensated per-block sum.  Arithmetic follows
-// CxLikExecuteStripeDna (CxLik.c:294-633) with FP32 storage.
+// 4-state traversal.  A block is one wave: 64 sites, one per lane, with all
+// C mixture components of a site in that lane's registers.  The wave walks
+// the whole plan (post-order list of fused node updates) for its sites: one
+// launch covers a full traversal and every evaluation of a batch, and no
+// wave waits for any other -- there is no barrier in the kernel.  Per update
+// and component the wave forms P_c * child_c for every child on the matrix
+// core (a tip contributes P_c * its 4-bit state mask, one byte per site),
+// multiplies them, takes the site maximum over all components and states in
+// registers, and rescales the site exactly by the power of two that brings it
+// into [0.5, 1), adding the exponent to the children's.  The scaled component
+// planes are stored to the HBM cache slot (layout [C][S][4], 1 KiB
+// contiguous per store) and, when the parent is in the same plan, pushed onto
+// the wave's LDS stack; parents pop children from there, so a full traversal
+// reads no CLV back from HBM.  The root update reduces to the
+// pattern-weighted FP64 site lnL (NaN -> -inf, CxLik.c:598-632) and a
+// compensated sum per wave.  Arithmetic follows CxLikExecuteStripeDna
+// (CxLik.c:294-633) with FP32 storage.
+//
+// Decoding an update, reading a tip mask and finding the site's exponent
+// are done once for all components, so the per-update control cost is shared
+// C ways.  Latency: every wave walks the plan in step with every other, so a
+// load issued when an update needs it meets the full L2 latency with nothing
+// to overlap it; and on this GPU one counter tracks loads and stores alike,
+// so a wave that waits for a load also waits for its own CLV stores to
+// drain.  Hence the P blocks and control words are prefetched into registers
+// a chunk of updates ahead and written to the wave's LDS double-buffered
+// (the chunk after next is fetched when a chunk starts and written just
+// before it is needed), the next update's control words and P rows are read
+// from LDS while the current one finishes, and an update reads all its
+// children's stack entries and tip masks at once before picking by kind.
+// Tip masks of up to kTipChunk tips are staged when the wave starts (four
+// sites of a row per lane and load); a plan with more tips stages the next
+// chunk where it starts.
+template <int CM>
+__host__ __device__ constexpr int trav4_pchunk()
+{
+    return CM <= 4 ? 8 : (CM == 8 ? 4 : 2);
+}
+
 template <int CM, int D>
-__global__ void __launch_bounds__(kTile4 * CM)
+__global__ void __launch_bounds__(kTile4)
 k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
+    constexpr int PC = trav4_pchunk<CM>();
+    constexpr int kChunkRows = PC * kOpMaxChild * CM * 4;   // float4 rows of P per chunk
     __shared__ float4 stk[D][CM][kTile4];
-    __shared__ int32_t stke[D][CM][kTile4];
-    __shared__ float smax[2][CM][kTile4];
-    __shared__ double sL[CM][kTile4];
-    __shared__ uint8_t tipm[kTipChunk][kTile4];
+    __shared__ int32_t stke[D][kTile4];
+    __shared__ uint32_t tipm[kTipChunk][kTile4 / 4];
+    __shared__ float4 pch[2][kChunkRows];
+    __shared__ int32_t cch[2][PC * kCtlWords];
     const int b = blk0 + (int)blockIdx.x;
     const CRUX_CAS Entry *E = cas(entries) + cas(block_entry)[b];
-    const int S = E->S, C = E->C, nops = E->nops;
+    const int S = E->S, C = E->C, nops = E->nops, ntips = E->ntips;
     const int tile = b - E->tile0;
-    const int lane = (int)threadIdx.x & (kTile4 - 1);
-    const int c = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTile4);
+    const int lane = (int)threadIdx.x;
     const int s = tile * kTile4 + lane;
     const bool act = s < S;
     const int sc = act ? s : S - 1;       // in-bounds site for loads of idle lanes
-    const bool wc = c < C;
     const CRUX_CAS Op *ops = cas(E->ops);
     double sum = 0.0;
 
-    // P blocks are staged into LDS kPChunk updates at a time, the next chunk
-    // being fetched while the current one runs: every block walks the plan
-    // in step with every other, so a per-update load would meet the full L2
-    // latency with nothing to overlap it.
-    __shared__ float pch[2][kPChunk * kOpMaxChild * CM * 16];
-    constexpr int kPer = kPChunk * kOpMaxChild * 16 / kTile4;   // floats per thread
-    const int tid = (int)threadIdx.x;
-    const int chunk_f = kPChunk * kOpMaxChild * C * 16;
-    const int total_f = nops * kOpMaxChild * C * 16;
-    const CRUX_GAS float *pd = gas(E->pdense);
-    // The updates' control words travel with the P chunks for the same reason.
-    __shared__ int32_t cch[2][kPChunk * kCtlWords];
+    // ---- P / control-word staging ---------------------------------------
+    constexpr int kPer = (kChunkRows + kTile4 - 1) / kTile4;   // float4 per lane
+    const int total_r = nops * kOpMaxChild * CM * 4;
+    const CRUX_GAS float4 *pd = gas(reinterpret_cast<const float4 *>(E->pdense));
     const CRUX_GAS int32_t *cg = gas(E->ctl);
-    float pf[kPer];
+    float4 pf[kPer];
     int32_t cf = 0;
     auto fetch = [&](int k) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const int i = tid * kPer + q, g = k * chunk_f + i;
-            pf[q] = (i < chunk_f && g < total_f) ? pd[g] : 0.0f;
+            const int i = q * kTile4 + lane, g = k * kChunkRows + i;
+            if (i < kChunkRows && g < total_r) {
+                const crux_f32x4 v = *(const CRUX_GAS crux_f32x4 *)(pd + g);
+                pf[q] = make_float4(v.x, v.y, v.z, v.w);
+            }
         }
-        if (tid < kPChunk * kCtlWords)
-            cf = (k * kPChunk + tid / kCtlWords < nops) ? cg[k * kPChunk * kCtlWords + tid] : 0;
+        if (lane < PC * kCtlWords)
+            cf = (k * PC + lane / kCtlWords < nops) ? cg[k * PC * kCtlWords + lane] : 0;
     };
     auto commit = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const int i = tid * kPer + q;
-            if (i < chunk_f)
+            const int i = q * kTile4 + lane;
+            if (i < kChunkRows)
                 pch[buf][i] = pf[q];
         }
-        if (tid < kPChunk * kCtlWords)
-            cch[buf][tid] = cf;
+        if (lane < PC * kCtlWords)
+            cch[buf][lane] = cf;
     };
+
+    // ---- tip staging: rows [base, base + n) of the entry's tip list --------
+    // One dword load covers four sites of a row, so an instruction covers
+    // four rows.  Mask rows are padded to whole tiles.
+    auto stage_tips = [&](int base) {
+        constexpr int kInstr = kTipChunk / 4;
+        const int n = min(kTipChunk, ntips - base);
+        const CRUX_CAS uint8_t *const CRUX_CAS *rows =
+            (const CRUX_CAS uint8_t *const CRUX_CAS *)cas(E->tip_rows) + base;
+        const int sub = lane >> 4;                            // row within the instruction
+        const int off = tile * kTile4 + (lane & 15) * 4;      // first site of the lane
+        uint32_t v[kInstr];
+#pragma unroll
+        for (int i = 0; i < kInstr; ++i) {
+            const int r = 4 * i + sub;
+            v[i] = 0u;
+            if (4 * i < n) {
+                const CRUX_CAS uint8_t *p0 = rows[4 * i];
+                const CRUX_CAS uint8_t *p1 = 4 * i + 1 < n ? rows[4 * i + 1] : p0;
+                const CRUX_CAS uint8_t *p2 = 4 * i + 2 < n ? rows[4 * i + 2] : p0;
+                const CRUX_CAS uint8_t *p3 = 4 * i + 3 < n ? rows[4 * i + 3] : p0;
+                const uint8_t *p = (const uint8_t *)(sub == 0 ? p0 : sub == 1 ? p1 : sub == 2 ? p2 : p3);
+                if (r < n)
+                    v[i] = *(const CRUX_GAS uint32_t *)gas(p + off);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kInstr; ++i) {
+            const int r = 4 * i + sub;
+            if (r < n)
+                tipm[r][lane & 15] = v[i];
+        }
+        // No load may stay outstanding past this point: the compiler would
+        // otherwise make later updates wait for their stores as well.
+        __builtin_amdgcn_s_waitcnt(0);
+    };
+
     fetch(0);
     commit(0);
-    fetch(1);
-    __syncthreads();
+    if (nops > PC)
+        fetch(1);
+    if (ntips > 0)
+        stage_tips(0);
+
+    // Control words and P rows (row lane % 4 of every child's and
+    // component's P) of the update about to run.
+    int4 cwa, cwb;
+    // Up to four components, an update's P rows and stack entries are read
+    // ahead into registers; with more, at their use (register budget).
+    constexpr bool kPre = CM <= 4;
+    float4 prow[kOpMaxChild][kPre ? CM : 1];
+    auto read_op = [&](int o) {
+        const int rb = (o / PC) & 1, rq = o % PC;
+        const int4 *cw = reinterpret_cast<const int4 *>(&cch[rb][rq * kCtlWords]);
+        cwa = cw[0];
+        cwb = cw[1];
+        // Components c >= C have all-zero P blocks, so every product below
+        // runs over all CM components without a branch and leaves them 0.
+        if constexpr (kPre) {
+#pragma unroll
+            for (int j = 0; j < kOpMaxChild; ++j)
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    prow[j][c] = pch[rb][((rq * kOpMaxChild + j) * CM + c) * 4 + (lane & 3)];
+        }
+    };
+    read_op(0);
+
+    // The previous update's scaled result and exponent: a child that is the
+    // result of the update just before is taken from here (it was never
+    // pushed).
+    float4 yl[CM];
+    int32_t el = 0;
+#pragma unroll
+    for (int c = 0; c < CM; ++c)
+        yl[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 
     for (int o = 0; o < nops; ++o) {
         const CRUX_CAS Op *op = ops + o;
-        if (o > 0 && o % kPChunk == 0) {
-            const int k = o / kPChunk;
-            commit(k & 1);
-            __syncthreads();
-            fetch(k + 1);
-        }
-        const int buf = (o / kPChunk) & 1;
-        const float *pchunk = &pch[buf][(o % kPChunk) * kOpMaxChild * C * 16];
-        const int32_t *cw = &cch[buf][(o % kPChunk) * kCtlWords];
-        const int w0 = __builtin_amdgcn_readfirstlane(cw[0]);
-        const int32_t pk = __builtin_amdgcn_readfirstlane(cw[1]);
-        const int tl = __builtin_amdgcn_readfirstlane(cw[2]);
-        const bool has_parent = __builtin_amdgcn_readfirstlane(cw[3]) != 0;
-        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xff;
-        if (tl >= 0) {
-            // Stage the next chunk of tip state masks; waves split the rows.
-            __syncthreads();
-            const int n = min(kTipChunk, E->ntips - tl);
-            const CRUX_CAS uint8_t *const CRUX_CAS *rows =
-                (const CRUX_CAS uint8_t *const CRUX_CAS *)cas(E->tip_rows) + tl;
-            for (int k = c; k < n; k += CM)
-                tipm[k][lane] = gas((const uint8_t *)rows[k])[sc];
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-        }
+        const int buf = (o / PC) & 1, oq = o % PC;
+        if (oq == 0 && o > 0 && o + PC < nops)
+            fetch(o / PC + 1);
+        const int w0 = __builtin_amdgcn_readfirstlane(cwa.x);
+        const int32_t pk = __builtin_amdgcn_readfirstlane(cwa.y);
+        const int tl = __builtin_amdgcn_readfirstlane(cwa.z);
+        const bool has_parent = __builtin_amdgcn_readfirstlane(cwa.w) != 0;
+        const uint64_t par = (uint32_t)__builtin_amdgcn_readfirstlane(cwb.x) |
+                             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cwb.y) << 32);
+        const uint64_t pex = (uint32_t)__builtin_amdgcn_readfirstlane(cwb.z) |
+                             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cwb.w) << 32);
+        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xf,
+                  fwd = (w0 >> 20) & 0xf;
+        if (tl > 0)
+            stage_tips(tl);
+        // The chunk after this one goes into the other buffer as this one
+        // ends.
+        if (oq == PC - 1 && o + 1 < nops)
+            commit(buf ^ 1);
 
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        int32_t ex = 0;
-        if (wc) {
+        // LDS reads of every child slot's stack entries and tip mask first,
+        // whatever the kinds (in bounds whatever the slot holds): they are in
+        // flight while the forwarded child is multiplied, and values defined
+        // on every path cost no register copies.
+        float4 sx[kOpMaxChild][kPre ? CM : 1];
+        int32_t se[kOpMaxChild];
+        uint32_t tm[kOpMaxChild];
 #pragma unroll
-            for (int j = 0; j < kOpMaxChild; ++j) {
-                if (j >= nch)
-                    break;
+        for (int j = 0; j < kOpMaxChild; ++j) {
+            const int p = op_pos(pk, j);
+            if constexpr (kPre) {
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    sx[j][c] = stk[p & (D - 1)][c][lane];
+            }
+            se[j] = stke[p & (D - 1)][lane];
+            tm[j] = tipm[p & (kTipChunk - 1)][lane >> 2];
+        }
+        auto prow_of = [&](int j, int c) -> float4 {
+            if constexpr (kPre)
+                return prow[j][c];
+            else
+                return pch[buf][((oq * kOpMaxChild + j) * CM + c) * 4 + (lane & 3)];
+        };
+        auto stack_of = [&](int j, int c) -> float4 {
+            if constexpr (kPre)
+                return sx[j][c];
+            else
+                return stk[op_pos(pk, j) & (D - 1)][c][lane];
+        };
+        float4 acc[CM];
+        int32_t ex = 0;
+        // acc = (first child ? v : acc * v), v = P_c x_c (matrix core, the
+        // components' chains interleaved; control flow is uniform across the
+        // wave here, as the matrix instructions need) or x_c (ident child).
+        auto take = [&](int j, auto &&xc, bool mat) {
+            if (mat) {
+                crux_f32x4 d[CM];
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).x, xc(c).x,
+                                                              crux_f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).y, xc(c).y, d[c], 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).z, xc(c).z, d[c], 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).w, xc(c).w, d[c], 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < CM; ++c) {
+                    const float4 v = make_float4(d[c].x, d[c].y, d[c].z, d[c].w);
+                    acc[c] = (j == 0) ? v : mul4(acc[c], v);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    acc[c] = (j == 0) ? xc(c) : mul4(acc[c], xc(c));
+            }
+        };
+#pragma unroll
+        for (int j = 0; j < kOpMaxChild; ++j) {
+            if (j >= nch)
+                break;
+            const int kind = (kinds >> (2 * j)) & 3;
+            const bool mat = kind != kChildIdent;
+            if (kind == kChildTip) {
+                const uint32_t m = tm[j] >> (8 * (lane & 3));
+                const float4 t = make_float4((float)(m & 1u), (float)((m >> 1) & 1u),
+                                             (float)((m >> 2) & 1u), (float)((m >> 3) & 1u));
+                take(j, [&](int) { return t; }, true);
+            } else if ((fwd >> j) & 1) {
+                take(j, [&](int c) { return yl[c]; }, mat);
+                ex += el;
+            } else if ((fst >> j) & 1) {
+                take(j, [&](int c) { return stack_of(j, c); }, mat);
+                ex += se[j];
+            } else {
+                // Cached by an earlier plan (partial re-evaluation).  The wait
+                // stays inside this branch (see stage_tips).
                 const CRUX_CAS OpChild &ch = op->ch[j];
-                const int kind = (kinds >> (2 * j)) & 3;
-                const int p = op_pos(pk, j);
-                float4 x;
-                if (kind == kChildTip) {
-                    const uint32_t m = tipm[p][lane];
-                    x = make_float4((float)(m & 1u), (float)((m >> 1) & 1u),
-                                    (float)((m >> 2) & 1u), (float)((m >> 3) & 1u));
-                } else if ((fst >> j) & 1) {
-                    x = stk[p][c][lane];
-                    ex += stke[p][c][lane];
-                } else {
-                    // Cached by an earlier plan (partial re-evaluation).
-                    x = ld4(gas(reinterpret_cast<const float4 *>(ch.clv)) + (size_t)c * S + sc);
-                    ex += gas(ch.exps)[sc];
-                    __builtin_amdgcn_s_waitcnt(0);
-                }
-                float4 v = x;
-                if (kind != kChildIdent) {
-                    const float4 *Pq = reinterpret_cast<const float4 *>(pchunk + (j * C + c) * 16);
-                    const float4 r0 = Pq[0], r1 = Pq[1], r2 = Pq[2], r3 = Pq[3];
-                    v.x = fmaf(r0.x, x.x, fmaf(r0.y, x.y, fmaf(r0.z, x.z, r0.w * x.w)));
-                    v.y = fmaf(r1.x, x.x, fmaf(r1.y, x.y, fmaf(r1.z, x.z, r1.w * x.w)));
-                    v.z = fmaf(r2.x, x.x, fmaf(r2.y, x.y, fmaf(r2.z, x.z, r2.w * x.w)));
-                    v.w = fmaf(r3.x, x.x, fmaf(r3.y, x.y, fmaf(r3.z, x.z, r3.w * x.w)));
-                }
-                acc = (j == 0) ? v : mul4(acc, v);
+                float4 x[CM];
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    x[c] = c < C ? ld4(gas(reinterpret_cast<const float4 *>(ch.clv)) +
+                                       (size_t)c * S + sc)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+                ex += gas(ch.exps)[sc];
+                __builtin_amdgcn_s_waitcnt(0);
+                take(j, [&](int c) { return x[c]; }, mat);
             }
         }
+        // The next update's control words and P rows, while this one ends.
+        if (o + 1 < nops)
+            read_op(o + 1);
         if (has_parent) {
-            smax[o & 1][c][lane] = wc ? max4(acc) : 0.0f;
-            __syncthreads();
-            float m = smax[o & 1][0][lane];
+            // Site maximum over every component (idle ones are 0).  All
+            // values are >= 0, so the largest as a signed integer is the
+            // largest float (NaN above +inf, -0 below everything).
+            int32_t mb = 0;
 #pragma unroll
-            for (int q = 1; q < CM; ++q)
-                if (q < C)
-                    m = fmaxf(m, smax[o & 1][q][lane]);
+            for (int c = 0; c < CM; ++c)
+                mb = max(max(mb, max(__float_as_int(acc[c].x), __float_as_int(acc[c].y))),
+                         max(__float_as_int(acc[c].z), __float_as_int(acc[c].w)));
+            const float m = __int_as_float(mb);
             int e2 = 0;
             if (m > 0.0f)
                 (void)frexpf(m, &e2);
-            if (wc) {
-                const float4 y = make_float4(ldexpf(acc.x, -e2), ldexpf(acc.y, -e2),
-                                             ldexpf(acc.z, -e2), ldexpf(acc.w, -e2));
-                const int pp = op_pos(pk, kOpMaxChild);
-                const uint64_t par = (uint32_t)__builtin_amdgcn_readfirstlane(cw[4]) |
-                                     ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[5]) << 32);
-                if (act)
-                    st4(gas(reinterpret_cast<float4 *>(par)) + (size_t)c * S + s, y);
-                if (pp >= 0) {
-                    stk[pp][c][lane] = y;
-                    stke[pp][c][lane] = ex + e2;
-                }
-                if (c == 0 && act) {
-                    const uint64_t pe = (uint32_t)__builtin_amdgcn_readfirstlane(cw[6]) |
-                                        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[7]) << 32);
-                    gas(reinterpret_cast<int32_t *>(pe))[s] = ex + e2;
-                }
+            el = ex + e2;
+#pragma unroll
+            for (int c = 0; c < CM; ++c)
+                yl[c] = make_float4(ldexpf(acc[c].x, -e2), ldexpf(acc[c].y, -e2),
+                                    ldexpf(acc[c].z, -e2), ldexpf(acc[c].w, -e2));
+            const int pp = op_pos(pk, kOpMaxChild);
+            if (pp >= 0) {
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    stk[pp][c][lane] = yl[c];
+                stke[pp][lane] = el;
+            }
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < CM; ++c)
+                    if (c < C)
+                        st4(gas(reinterpret_cast<float4 *>(par)) + (size_t)c * S + s, yl[c]);
+                gas(reinterpret_cast<int32_t *>(pex))[s] = el;
             }
         } else {
-            double Lc = 0.0;
-            if (wc) {
-                const CRUX_CAS double *piw = cas(E->piw) + 4 * c;
-                Lc = piw[0] * (double)acc.x + piw[1] * (double)acc.y +
-                     piw[2] * (double)acc.z + piw[3] * (double)acc.w;
-            }
-            sL[c][lane] = Lc;
-            __syncthreads();
-            if (c == 0) {
-                double L = 0.0;
+            double L = 0.0;
 #pragma unroll
-                for (int q = 0; q < CM; ++q)
-                    if (q < C)
-                        L += sL[q][lane];
-                double v = (log(L) + (double)ex * 0.69314718055994530942) *
-                           (double)gas(E->weights)[sc];
-                if (__builtin_isnan(v))
-                    v = -INFINITY;
-                if (act) {
-                    gas(E->site_out)[s] = v;
-                    sum = v;
+            for (int c = 0; c < CM; ++c) {
+                if (c < C) {
+                    const CRUX_CAS double *piw = cas(E->piw) + 4 * c;
+                    L += piw[0] * (double)acc[c].x + piw[1] * (double)acc[c].y +
+                         piw[2] * (double)acc[c].z + piw[3] * (double)acc[c].w;
                 }
+            }
+            double v = (log(L) + (double)ex * 0.69314718055994530942) *
+                       (double)gas(E->weights)[sc];
+            if (__builtin_isnan(v))
+                v = -INFINITY;
+            if (act) {
+                gas(E->site_out)[s] = v;
+                sum = v;
             }
         }
     }
-    if (c != 0)
-        return;
     // Compensated wave reduction: both lanes of a pair merge in the same order.
     double cmp = 0.0;
 #pragma unroll
@@ -637,7 +751,7 @@ static void launch_trav4(const Entry *entries, const int32_t *block_entry, int b
                          int nblocks, hipStream_t stream)
 {
     if constexpr (D <= stack_depth4_max(CM))
-        hipLaunchKernelGGL((k_trav4<CM, D>), dim3(nblocks), dim3(kTile4 * CM), 0, stream,
+        hipLaunchKernelGGL((k_trav4<CM, D>), dim3(nblocks), dim3(kTile4), 0, stream,
                            entries, block_entry, blk0);
 }
 
