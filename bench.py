@@ -172,6 +172,17 @@ def main():
                          "GTR+G4 workload by the FP64 C restatement of the reference "
                          "engine (oracle/cxlik_oracle.c) with %d stripe threads" % (n, S, ncpu)}
 
+    # HBM bytes per traversal launch as rocprofv3's FETCH_SIZE + WRITE_SIZE
+    # counters read them for this workload (scaled by the copy-kernel
+    # calibration), recorded next to the profile summary they come from.
+    traffic_hbm = None
+    prof = os.path.join(ROOT, "profiles", "k_trav4_dna_c2.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            rec = json.load(f)
+        if rec["workload"] == {"taxa": 128, "patterns": S, "categories": 4}:
+            traffic_hbm = int(rec["traffic_bytes"])
+
     line = {
         "metric": "lnL evals/s (full traversal, 128 taxa x ~60k patterns GTR+G4)",
         "value": round(full_eps, 2),
@@ -197,7 +208,7 @@ def main():
                       "finalize": round(float(tm[2]), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes": traffic, "node_updates": nodes,
+                     "traffic": traffic_hbm, "algorithmic_bytes": traffic, "node_updates": nodes,
                      "model_bytes": formula_bytes,
                      "model_GBps": round(formula_bytes / (trav_ms * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
