@@ -40,6 +40,8 @@ SIGNATURES = {
     "crux_tree_set_topology": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
                                               _i32p, _i32p, _dp]),
     "crux_tree_set_lengths": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp]),
+    "crux_tree_update_rings": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p,
+                                              ctypes.c_int, _i32p, _i32p]),
     "crux_lik_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
     "crux_lik_destroy": (ctypes.c_int, [_vp]),
     "crux_lik_set_model": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,

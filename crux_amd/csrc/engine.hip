@@ -1278,6 +1278,85 @@ int crux_tree_set_topology(crux_tree *t, int nnodes, int nedges, const int32_t *
     return CRUX_OK;
 }
 
+int crux_tree_update_rings(crux_tree *t, int nrings, const int32_t *rings,
+                           const int32_t *ring_node, const int32_t *ring_next, int nnodes,
+                           const int32_t *nodes, const int32_t *node_ring)
+{
+    if (!t || !t->has_topology || nrings < 0 || nnodes < 0 ||
+        (nrings > 0 && (!rings || !ring_node || !ring_next)) ||
+        (nnodes > 0 && (!nodes || !node_ring)))
+        return set_error(CRUX_EINVAL, "bad ring update arguments");
+    const int nr = 2 * t->nedges;
+    for (int i = 0; i < nrings; i++)
+        if (rings[i] < 0 || rings[i] >= nr || ring_node[i] < -1 || ring_node[i] >= t->nnodes ||
+            ring_next[i] < -1 || ring_next[i] >= nr || (ring_node[i] >= 0) != (ring_next[i] >= 0))
+            return set_error(CRUX_EINVAL, "bad ring update %d", i);
+    for (int i = 0; i < nnodes; i++)
+        if (nodes[i] < 0 || nodes[i] >= t->nnodes || node_ring[i] < -1 || node_ring[i] >= nr)
+            return set_error(CRUX_EINVAL, "bad node update %d", i);
+    // Apply to copies, check every touched node, then commit.
+    std::vector<int32_t> rn = t->ring_node, rx = t->ring_next, nring = t->node_ring;
+    std::vector<int32_t> touched;
+    for (int i = 0; i < nrings; i++) {
+        if (rn[rings[i]] >= 0)
+            touched.push_back(rn[rings[i]]);
+        rn[rings[i]] = ring_node[i];
+        rx[rings[i]] = ring_next[i];
+        if (ring_node[i] >= 0)
+            touched.push_back(ring_node[i]);
+    }
+    for (int i = 0; i < nnodes; i++) {
+        nring[nodes[i]] = node_ring[i];
+        touched.push_back(nodes[i]);
+    }
+    for (int i = 0; i < nrings; i++) {
+        const int32_t r = rings[i];
+        if ((rn[r] >= 0) != (rn[r ^ 1] >= 0))
+            return set_error(CRUX_EINVAL, "ring %d attached at one end only", r);
+        if (rn[r] >= 0 && rn[rx[r]] != rn[r])
+            return set_error(CRUX_EINVAL, "ring %d: next ring on another node", r);
+    }
+    std::sort(touched.begin(), touched.end());
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    std::vector<std::pair<int32_t, int>> deg;
+    for (int32_t n : touched) {
+        const int32_t h = nring[n];
+        int cnt = 0;
+        if (h >= 0) {
+            if (rn[h] != n)
+                return set_error(CRUX_EINVAL, "bad node ring %d", n);
+            int32_t r = h;
+            do {
+                if (rn[r] != n || ++cnt > nr)
+                    return set_error(CRUX_EINVAL, "ring cycle of node %d is broken", n);
+                r = rx[r];
+            } while (r != h);
+        }
+        deg.push_back({n, cnt});
+    }
+    // Every ring that claims a touched node must lie on its cycle.
+    for (int i = 0; i < nrings; i++) {
+        const int32_t n = rn[rings[i]];
+        if (n < 0)
+            continue;
+        int32_t r = nring[n];
+        bool found = false;
+        if (r >= 0)
+            do {
+                found |= r == rings[i];
+                r = rx[r];
+            } while (r != nring[n] && !found);
+        if (!found)
+            return set_error(CRUX_EINVAL, "ring %d is not on node %d's cycle", rings[i], n);
+    }
+    t->ring_node.swap(rn);
+    t->ring_next.swap(rx);
+    t->node_ring.swap(nring);
+    for (auto &d : deg)
+        t->degree[d.first] = d.second;
+    return CRUX_OK;
+}
+
 int crux_tree_set_lengths(crux_tree *t, int n, const int32_t *edges, const double *lens)
 {
     if (!t || n < 0 || (n > 0 && (!edges || !lens)))

@@ -94,20 +94,40 @@ class _DeviceTree:
             nat.ctypes.byref(h)))
         self.handle = h
         self.sn = None
+        self.ids = None
         self.sink = tree.length_sink()
+        self.tsink = tree.topology_sink()
 
     def sync(self):
         t = self.tree
         # Only attach/detach change what the device mirrors (moving the base
-        # does not), so key the full upload on the topology serial.
+        # does not), so key the topology upload on the topology serial; when
+        # no node or edge was created since the last upload, only the rings
+        # and nodes that attach/detach touched are sent.
         if self.sn != t.tsn:
-            rn, rx, nr, nt, el = t.topology_arrays(self.alignment.taxaMap)
-            nat.check(nat.lib().crux_tree_set_topology(
-                self.handle, len(nr), len(el), nat.iptr(rn), nat.iptr(rx), nat.iptr(nr),
-                nat.iptr(nt), nat.dptr(el)))
+            rings, nodes = self.tsink
+            if self.ids == (t.max_node_id, t.max_edge_id):
+                rl = sorted(rings, key=lambda r: r.id)
+                nl = sorted(nodes, key=lambda n: n.id)
+                ids = nat.i32([r.id for r in rl])
+                rnode = nat.i32([r.node.id if r.node is not None else -1 for r in rl])
+                rnext = nat.i32([r.next.id if r.node is not None else -1 for r in rl])
+                nid = nat.i32([n.id for n in nl])
+                nring = nat.i32([n.ring.id if n.ring is not None else -1 for n in nl])
+                nat.check(nat.lib().crux_tree_update_rings(
+                    self.handle, len(ids), nat.iptr(ids), nat.iptr(rnode), nat.iptr(rnext),
+                    len(nid), nat.iptr(nid), nat.iptr(nring)))
+            else:
+                rn, rx, nr, nt, el = t.topology_arrays(self.alignment.taxaMap)
+                nat.check(nat.lib().crux_tree_set_topology(
+                    self.handle, len(nr), len(el), nat.iptr(rn), nat.iptr(rx), nat.iptr(nr),
+                    nat.iptr(nt), nat.dptr(el)))
+                self.ids = (t.max_node_id, t.max_edge_id)
+                self.sink.clear()
+            rings.clear()
+            nodes.clear()
             self.sn = t.tsn
-            self.sink.clear()
-        elif self.sink:
+        if self.sink:
             edges = list(self.sink)
             self.sink.clear()
             ids = nat.i32([e.id for e in edges])

@@ -142,6 +142,7 @@ class Edge:
                 head.prev = ring
                 prev.next = ring
             node.ring = ring
+            self.tree._topo_touch(ring, ring.prev, node)
         self.tree.sn += 1
         self.tree.tsn += 1
 
@@ -151,6 +152,7 @@ class Edge:
             nxt = ring.next
             if nxt is ring:
                 node.ring = None
+                self.tree._topo_touch(ring, None, node)
             else:
                 if node.ring is ring:
                     node.ring = nxt
@@ -159,6 +161,7 @@ class Edge:
                 prev.next = nxt
                 ring.next = ring
                 ring.prev = ring
+                self.tree._topo_touch(ring, prev, node)
             ring.node = None
         self.tree.sn += 1
         self.tree.tsn += 1
@@ -285,6 +288,7 @@ class Tree:
         self.tsn = 0         # serial number of attach/detach only
         self.lsn = 0         # branch-length serial number
         self._sinks = []     # sets that collect edges whose length changed
+        self._topo_sinks = []   # (rings, nodes) touched by attach/detach
         self.aux = None
         self._cache_sn = -1
         self._nodes = []
@@ -328,6 +332,21 @@ class Tree:
         s = set()
         self._sinks.append(s)
         return s
+
+    def topology_sink(self):
+        """A pair of sets (rings, nodes) that from now on collects every ring
+        and node whose links attach/detach change (device mirrors send just
+        those)."""
+        s = (set(), set())
+        self._topo_sinks.append(s)
+        return s
+
+    def _topo_touch(self, ring, prev, node):
+        for rings, nodes in self._topo_sinks:
+            rings.add(ring)
+            if prev is not None:
+                rings.add(prev)
+            nodes.add(node)
 
     def _label(self, node, label):
         node.taxon = label

@@ -109,6 +109,16 @@ int crux_tree_set_topology(crux_tree *tree, int nnodes, int nedges,
 /* Update some branch lengths (Edge.length writes). */
 int crux_tree_set_lengths(crux_tree *tree, int n, const int32_t *edges,
                           const double *lens);
+/* Partial topology update after Edge.attach/detach calls (the ring list
+ * edits of pkg/Crux/Tree/__init__.pyx Edge.attach/detach): new ring_node and
+ * ring_next of nrings rings, new node_ring of nnodes nodes, in the encoding
+ * of crux_tree_set_topology; node and ring counts are unchanged.  The ring
+ * cycles of every node touched are checked as crux_tree_set_topology does.
+ * Cached CLVs are kept; their cache keys decide what is recomputed, as after
+ * a full topology upload. */
+int crux_tree_update_rings(crux_tree *tree, int nrings, const int32_t *rings,
+                           const int32_t *ring_node, const int32_t *ring_next,
+                           int nnodes, const int32_t *nodes, const int32_t *node_ring);
 
 /* ---- likelihoods ---------------------------------------------------------- */
 
