@@ -50,6 +50,9 @@ SIGNATURES = {
     "crux_lik_site_lnl": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
     "crux_lik_flush": (ctypes.c_int, [_vp]),
     "crux_batch_lnl": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _i32p, _dp]),
+    "crux_batch_submit": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _i32p,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "crux_batch_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, _dp]),
     "crux_ctx_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "crux_ctx_get_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
@@ -225,4 +228,22 @@ def batch_lnl(ctx, liks, roots):
     r = i32(roots)
     out = np.empty(n)
     check(lib().crux_batch_lnl(ctx.handle, n, arr, iptr(r), dptr(out)))
+    return out
+
+
+def batch_submit(ctx, liks, roots):
+    """Plan and enqueue a fused pass; returns (ticket, n) for batch_wait."""
+    n = len(liks)
+    arr = (_vp * max(1, n))(*[l.handle for l in liks])
+    r = i32(roots)
+    t = ctypes.c_uint64()
+    check(lib().crux_batch_submit(ctx.handle, n, arr, iptr(r), ctypes.byref(t)))
+    return (t.value, n)
+
+
+def batch_wait(ctx, ticket):
+    """Results of a submitted pass (blocks until it is done)."""
+    t, n = ticket
+    out = np.empty(n)
+    check(lib().crux_batch_wait(ctx.handle, t, n, dptr(out)))
     return out

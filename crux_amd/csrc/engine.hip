@@ -76,18 +76,35 @@ struct crux_ctx {
     uint64_t last_bytes = 0;      // HBM bytes the last evaluation had to move
     uint64_t last_formula = 0;    // same with every child CLV read from HBM
     int last_nodes = 0;
-    // Staging: one pinned host buffer mirrored by one device buffer.
-    char *h_plan = nullptr;
-    char *d_plan = nullptr;
-    size_t plan_cap = 0;
-    float *d_pmat = nullptr;       // P blocks and tip tables
-    size_t pmat_cap = 0;           // floats
-    double *d_partials = nullptr;  // [sum ntiles][2]
-    size_t partials_cap = 0;       // doubles
-    double *d_lnl = nullptr;
-    double *h_lnl = nullptr;
-    size_t lnl_cap = 0;
-    std::mutex mu;                 // one execution at a time per context
+    // Two staging sets, so the host plans and uploads one batch while the
+    // device runs the previous one (crux_batch_submit / crux_batch_wait).
+    // Batches run in submission order on the one stream, which is what makes
+    // a tree's cache bookkeeping (done when a batch is planned) valid for the
+    // next batch before the previous one has finished.
+    struct Staging {
+        char *h_plan = nullptr;        // pinned, mirrored by d_plan
+        char *d_plan = nullptr;
+        size_t plan_cap = 0;
+        float *d_pmat = nullptr;       // P blocks and tip tables
+        size_t pmat_cap = 0;           // floats
+        double *d_partials = nullptr;  // [sum ntiles][2]
+        size_t partials_cap = 0;       // doubles
+        double *d_lnl = nullptr;
+        double *h_lnl = nullptr;       // pinned
+        size_t lnl_cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+        bool timed = false;
+        uint64_t ticket = 0;
+        std::vector<crux_lik *> liks;                  // the batch, in call order
+        std::vector<std::pair<crux_lik *, int>> items; // evaluated: (lik, index)
+    } stg[2];
+    int cur = 0;                   // slot the next batch uses
+    uint64_t next_ticket = 1;
+    std::unordered_map<uint64_t, std::vector<double>> results;  // completed, not collected
+    int ntrees = 0;                // trees alive on this context
+    bool dying = false;            // destroy requested while trees were alive
+    std::mutex mu;                 // one host thread in the context at a time
 };
 
 static int dev_alloc(crux_ctx *ctx, void **p, size_t bytes)
@@ -649,8 +666,73 @@ static std::vector<int32_t> assign_tips(std::vector<PlanOp> &ops, bool per_launc
     return taxa;
 }
 
-static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
+using Staging = crux_ctx::Staging;
+
+// Finish the batch in staging slot k (and first any older batch in the other
+// slot): wait for its event, hand the results to the likelihoods, keep them
+// under the batch's ticket for crux_batch_wait.
+static int complete_slot(crux_ctx *ctx, int k)
 {
+    Staging &S = ctx->stg[k];
+    if (!S.pending)
+        return CRUX_OK;
+    Staging &O = ctx->stg[k ^ 1];
+    if (O.pending && O.ticket < S.ticket) {
+        int rc = complete_slot(ctx, k ^ 1);
+        if (rc)
+            return rc;
+    }
+    S.pending = false;
+    HIP_TRY(hipEventSynchronize(S.done));
+    if (S.timed) {
+        float a = 0, b = 0, c = 0, d = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[3]));
+        ctx->last_ms[0] = a;
+        ctx->last_ms[1] = b;
+        ctx->last_ms[2] = c;
+        ctx->last_ms[3] = d;
+    }
+    for (auto &it : S.items) {
+        it.first->last_lnl = S.h_lnl[it.second];
+        it.first->have_lnl = true;
+    }
+    std::vector<double> r(S.liks.size());
+    for (size_t i = 0; i < S.liks.size(); i++)
+        r[i] = S.liks[i]->last_lnl;
+    ctx->results[S.ticket] = std::move(r);
+    S.liks.clear();
+    S.items.clear();
+    return CRUX_OK;
+}
+
+static int drain(crux_ctx *ctx)
+{
+    while (ctx->stg[0].pending || ctx->stg[1].pending) {
+        const int k = ctx->stg[0].pending &&
+                              (!ctx->stg[1].pending || ctx->stg[0].ticket < ctx->stg[1].ticket)
+                          ? 0
+                          : 1;
+        int rc = complete_slot(ctx, k);
+        if (rc)
+            return rc;
+    }
+    return CRUX_OK;
+}
+
+// Plan and enqueue one batch of planned likelihoods; the results are
+// collected with collect() under the returned ticket.
+static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket)
+{
+    const int slot = ctx->cur;
+    if (ctx->stg[slot].pending) {
+        int rc0 = complete_slot(ctx, slot);
+        if (rc0)
+            return rc0;
+    }
+    Staging &S = ctx->stg[slot];
     // Classes by (K, CM) for the traversal launches.
     struct Item {
         crux_lik *L;
@@ -673,8 +755,16 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     ctx->last_formula = 0;
     ctx->last_nodes = 0;
     if (items.empty()) {
+        // Nothing to run: the values are the last ones, once any pending
+        // batch has delivered them.
+        int rc0 = drain(ctx);
+        if (rc0)
+            return rc0;
+        std::vector<double> r(liks.size());
         for (size_t i = 0; i < liks.size(); i++)
-            out[i] = liks[i]->last_lnl;
+            r[i] = liks[i]->last_lnl;
+        *ticket = ctx->next_ticket++;
+        ctx->results[*ticket] = std::move(r);
         return CRUX_OK;
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
@@ -741,42 +831,42 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     if (shmem > 64 * 1024)
         return set_error(CRUX_EINVAL, "P-matrix job too large for LDS");
 
-    if (plan_bytes > ctx->plan_cap) {
-        size_t ncap = std::max(plan_bytes, ctx->plan_cap * 2);
-        if (ctx->h_plan)
-            (void)hipHostFree(ctx->h_plan);
-        dev_free(ctx, ctx->d_plan, ctx->plan_cap);
-        ctx->h_plan = nullptr;
-        ctx->d_plan = nullptr;
-        ctx->plan_cap = 0;
-        HIP_TRY(hipHostMalloc((void **)&ctx->h_plan, ncap, hipHostMallocDefault));
-        int rc = dev_alloc(ctx, (void **)&ctx->d_plan, ncap);
+    if (plan_bytes > S.plan_cap) {
+        size_t ncap = std::max(plan_bytes, S.plan_cap * 2);
+        if (S.h_plan)
+            (void)hipHostFree(S.h_plan);
+        dev_free(ctx, S.d_plan, S.plan_cap);
+        S.h_plan = nullptr;
+        S.d_plan = nullptr;
+        S.plan_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&S.h_plan, ncap, hipHostMallocDefault));
+        int rc = dev_alloc(ctx, (void **)&S.d_plan, ncap);
         if (rc)
             return rc;
-        ctx->plan_cap = ncap;
+        S.plan_cap = ncap;
     }
-    int rc = grow(ctx, &ctx->d_pmat, &ctx->pmat_cap, std::max<size_t>(pm_floats, 64));
+    int rc = grow(ctx, &S.d_pmat, &S.pmat_cap, std::max<size_t>(pm_floats, 64));
     if (rc)
         return rc;
-    rc = grow(ctx, &ctx->d_partials, &ctx->partials_cap, std::max<size_t>(2 * tiles, 2));
+    rc = grow(ctx, &S.d_partials, &S.partials_cap, std::max<size_t>(2 * tiles, 2));
     if (rc)
         return rc;
-    if (liks.size() > ctx->lnl_cap) {
-        size_t ncap = std::max(liks.size(), ctx->lnl_cap * 2);
-        if (ctx->h_lnl)
-            (void)hipHostFree(ctx->h_lnl);
-        dev_free(ctx, ctx->d_lnl, ctx->lnl_cap * sizeof(double));
-        ctx->h_lnl = nullptr;
-        ctx->d_lnl = nullptr;
-        ctx->lnl_cap = 0;
-        HIP_TRY(hipHostMalloc((void **)&ctx->h_lnl, ncap * sizeof(double), hipHostMallocDefault));
-        rc = dev_alloc(ctx, (void **)&ctx->d_lnl, ncap * sizeof(double));
+    if (liks.size() > S.lnl_cap) {
+        size_t ncap = std::max(liks.size(), S.lnl_cap * 2);
+        if (S.h_lnl)
+            (void)hipHostFree(S.h_lnl);
+        dev_free(ctx, S.d_lnl, S.lnl_cap * sizeof(double));
+        S.h_lnl = nullptr;
+        S.d_lnl = nullptr;
+        S.lnl_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&S.h_lnl, ncap * sizeof(double), hipHostMallocDefault));
+        rc = dev_alloc(ctx, (void **)&S.d_lnl, ncap * sizeof(double));
         if (rc)
             return rc;
-        ctx->lnl_cap = ncap;
+        S.lnl_cap = ncap;
     }
 
-    char *H = ctx->h_plan, *D = ctx->d_plan;
+    char *H = S.h_plan, *D = S.d_plan;
     Entry *hent = (Entry *)(H + off_ent);
     Op *hops = (Op *)(H + off_ops);
     PJob *hjobs = (PJob *)(H + off_jobs);
@@ -873,9 +963,9 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
                     jb.ncodes = t->ncodes;
                     jb.cm = K == 4 && !jb.tip ? CM : C;
                     if (K == 4) {
-                        jb.out = ctx->d_pmat + pdense + (olocal * kOpMaxChild + j) * CM * 16;
+                        jb.out = S.d_pmat + pdense + (olocal * kOpMaxChild + j) * CM * 16;
                     } else {
-                        jb.out = ctx->d_pmat + ipm;
+                        jb.out = S.d_pmat + ipm;
                         ipm += align_up(jb.tip ? (size_t)t->ncodes * C * K : (size_t)C * K * K, 64);
                     }
                     oc.mat = jb.out;
@@ -894,11 +984,11 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
         base.piw = L->d_piw;
         base.weights = t->d_weights;
         base.site_out = L->d_site;
-        base.partials = ctx->d_partials + 2 * itile;
-        base.lnl_out = ctx->d_lnl + items[ii].idx;
+        base.partials = S.d_partials + 2 * itile;
+        base.lnl_out = S.d_lnl + items[ii].idx;
         base.ntips = (int32_t)items[ii].tips.size();
         base.tip_rows = nullptr;
-        base.pdense = K == 4 ? ctx->d_pmat + pdense : nullptr;
+        base.pdense = K == 4 ? S.d_pmat + pdense : nullptr;
         if (!items[ii].tips.empty()) {
             const uint8_t **hrows = (const uint8_t **)(H + off_tips) + itip;
             for (size_t k = 0; k < items[ii].tips.size(); k++)
@@ -976,30 +1066,57 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
     HIP_TRY(hipGetLastError());
     if (ctx->timing)
         HIP_TRY(hipEventRecord(ctx->ev[3], st));
-    HIP_TRY(hipMemcpyAsync(ctx->h_lnl, ctx->d_lnl, liks.size() * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(S.h_lnl, S.d_lnl, liks.size() * sizeof(double),
                            hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (ctx->timing) {
-        float a = 0, b = 0, c = 0, d = 0;
-        HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
-        HIP_TRY(hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[3]));
-        ctx->last_ms[0] = a;
-        ctx->last_ms[1] = b;
-        ctx->last_ms[2] = c;
-        ctx->last_ms[3] = d;
-    }
+    HIP_TRY(hipEventRecord(S.done, st));
     ctx->last_bytes = traffic;
     ctx->last_formula = formula;
     ctx->last_nodes = nodes;
-    for (Item &it : items) {
-        it.L->last_lnl = ctx->h_lnl[it.idx];
-        it.L->have_lnl = true;
-    }
-    for (size_t i = 0; i < liks.size(); i++)
-        out[i] = liks[i]->last_lnl;
+    S.pending = true;
+    S.timed = ctx->timing;
+    S.ticket = ctx->next_ticket++;
+    S.liks = liks;
+    S.items.clear();
+    for (Item &it : items)
+        S.items.push_back({it.L, it.idx});
+    ctx->cur = slot ^ 1;
+    *ticket = S.ticket;
     return CRUX_OK;
+}
+
+// Results of a submitted batch, waiting for it if needed.
+static int collect(crux_ctx *ctx, uint64_t ticket, double *out, size_t n)
+{
+    auto f = ctx->results.find(ticket);
+    if (f == ctx->results.end()) {
+        for (int k = 0; k < 2; k++)
+            if (ctx->stg[k].pending && ctx->stg[k].ticket == ticket) {
+                int rc = complete_slot(ctx, k);
+                if (rc)
+                    return rc;
+            }
+        f = ctx->results.find(ticket);
+        if (f == ctx->results.end())
+            return set_error(CRUX_EINVAL, "unknown or collected batch ticket %llu",
+                             (unsigned long long)ticket);
+    }
+    if (f->second.size() != n) {
+        ctx->results.erase(f);
+        return set_error(CRUX_EINVAL, "batch ticket %llu holds %zu results, not %zu",
+                         (unsigned long long)ticket, f->second.size(), n);
+    }
+    std::copy(f->second.begin(), f->second.end(), out);
+    ctx->results.erase(f);
+    return CRUX_OK;
+}
+
+static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
+{
+    uint64_t t = 0;
+    int rc = submit(ctx, liks, &t);
+    if (rc)
+        return rc;
+    return collect(ctx, t, out, liks.size());
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,6 +1157,8 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; i++)
         e = hipEventCreate(&ctx->ev[i]);
+    for (int k = 0; k < 2 && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&ctx->stg[k].done, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete ctx;
         return set_error(CRUX_EDEVICE, "context setup: %s", hipGetErrorString(e));
@@ -1052,16 +1171,26 @@ int crux_ctx_destroy(crux_ctx *ctx)
 {
     if (!ctx)
         return CRUX_OK;
+    if (ctx->ntrees > 0) {
+        // Trees still refer to the context (and their liks to the trees):
+        // the last crux_tree_destroy finishes the job.
+        ctx->dying = true;
+        return CRUX_OK;
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->h_plan)
-        (void)hipHostFree(ctx->h_plan);
-    if (ctx->h_lnl)
-        (void)hipHostFree(ctx->h_lnl);
-    dev_free(ctx, ctx->d_plan, ctx->plan_cap);
-    dev_free(ctx, ctx->d_pmat, ctx->pmat_cap * sizeof(float));
-    dev_free(ctx, ctx->d_partials, ctx->partials_cap * sizeof(double));
-    dev_free(ctx, ctx->d_lnl, ctx->lnl_cap * sizeof(double));
+    for (Staging &S : ctx->stg) {
+        if (S.h_plan)
+            (void)hipHostFree(S.h_plan);
+        if (S.h_lnl)
+            (void)hipHostFree(S.h_lnl);
+        dev_free(ctx, S.d_plan, S.plan_cap);
+        dev_free(ctx, S.d_pmat, S.pmat_cap * sizeof(float));
+        dev_free(ctx, S.d_partials, S.partials_cap * sizeof(double));
+        dev_free(ctx, S.d_lnl, S.lnl_cap * sizeof(double));
+        if (S.done)
+            (void)hipEventDestroy(S.done);
+    }
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i])
             (void)hipEventDestroy(ctx->ev[i]);
@@ -1197,6 +1326,7 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
         crux_tree_destroy(t);
         return set_error(CRUX_EDEVICE, "tree upload: %s", hipGetErrorString(e));
     }
+    ctx->ntrees++;
     *out = t;
     return CRUX_OK;
 }
@@ -1206,6 +1336,10 @@ int crux_tree_destroy(crux_tree *t)
     if (!t)
         return CRUX_OK;
     crux_ctx *ctx = t->ctx;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        (void)drain(ctx);
+    }
     (void)hipStreamSynchronize(ctx->stream);
     arena_reset(t, 0);
     arena_reset(t, 1);
@@ -1217,6 +1351,8 @@ int crux_tree_destroy(crux_tree *t)
         if (t->owner[p])
             t->owner[p]->tree = nullptr;
     delete t;
+    if (--ctx->ntrees == 0 && ctx->dying)
+        crux_ctx_destroy(ctx);
     return CRUX_OK;
 }
 
@@ -1392,6 +1528,11 @@ int crux_lik_destroy(crux_lik *L)
     crux_tree *t = L->tree;
     if (t) {
         crux_ctx *ctx = t->ctx;
+        {
+            // Submitted batches may still read this lik's buffers.
+            std::lock_guard<std::mutex> g(ctx->mu);
+            (void)drain(ctx);
+        }
         (void)hipStreamSynchronize(ctx->stream);
         if (t->owner[L->pol] == L)
             t->owner[L->pol] = nullptr;
@@ -1520,13 +1661,13 @@ int crux_lik_flush(crux_lik *L)
     return CRUX_OK;
 }
 
-int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t *roots,
-                   double *out)
+}  // extern "C"
+
+// Validate and plan one batch (caller holds ctx->mu).
+static int batch_plan(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t *roots,
+                      std::vector<crux_lik *> &v)
 {
-    if (!ctx || n < 0 || (n > 0 && (!liks || !roots || !out)))
-        return set_error(CRUX_EINVAL, "bad arguments");
-    std::lock_guard<std::mutex> g(ctx->mu);
-    std::vector<crux_lik *> v(liks, liks + n);
+    v.assign(liks, liks + n);
     // Each tree-polarity may appear once: the plan mutates its cache state.
     for (int i = 0; i < n; i++) {
         if (!v[i] || !v[i]->tree || v[i]->tree->ctx != ctx)
@@ -1543,7 +1684,47 @@ int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t *r
             return set_error(rc, "lik %d: %s", i, crux_last_error());
         }
     }
-    int rc = execute(ctx, v, out);
+    return CRUX_OK;
+}
+
+extern "C" {
+
+int crux_batch_submit(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t *roots,
+                      uint64_t *ticket)
+{
+    if (!ctx || n < 0 || !ticket || (n > 0 && (!liks || !roots)))
+        return set_error(CRUX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    std::vector<crux_lik *> v;
+    int rc = batch_plan(ctx, n, liks, roots, v);
+    if (rc)
+        return rc;
+    rc = submit(ctx, v, ticket);
+    if (rc)
+        for (crux_lik *L : v)
+            L->invalidate = true;
+    return rc;
+}
+
+int crux_batch_wait(crux_ctx *ctx, uint64_t ticket, int n, double *out)
+{
+    if (!ctx || n < 0 || (n > 0 && !out))
+        return set_error(CRUX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return collect(ctx, ticket, out, (size_t)n);
+}
+
+int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t *roots,
+                   double *out)
+{
+    if (!ctx || n < 0 || (n > 0 && (!liks || !roots || !out)))
+        return set_error(CRUX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    std::vector<crux_lik *> v;
+    int rc = batch_plan(ctx, n, liks, roots, v);
+    if (rc)
+        return rc;
+    rc = execute(ctx, v, out);
     if (rc)
         for (crux_lik *L : v)
             L->invalidate = true;

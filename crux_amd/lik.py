@@ -128,17 +128,23 @@ class _DeviceTree:
             nodes.clear()
             self.sn = t.tsn
         if self.sink:
+            # Usually one or two edges (an MC3 move): plain ctypes arrays
+            # cost less than numpy ones at that size.
             edges = list(self.sink)
             self.sink.clear()
-            ids = nat.i32([e.id for e in edges])
-            lens = nat.f64([e.length for e in edges])
-            nat.check(nat.lib().crux_tree_set_lengths(self.handle, len(ids), nat.iptr(ids),
-                                                      nat.dptr(lens)))
+            n = len(edges)
+            ids = (nat.ctypes.c_int32 * n)(*[e.id for e in edges])
+            lens = (nat.ctypes.c_double * n)(*[e.length for e in edges])
+            nat.check(nat.lib().crux_tree_set_lengths(self.handle, n, ids, lens))
 
     def close(self):
         if self.handle:
             nat.lib().crux_tree_destroy(self.handle)
             self.handle = None
+            # Stop collecting edits for a mirror that no longer exists.
+            t = self.tree
+            t._sinks = [x for x in t._sinks if x is not self.sink]
+            t._topo_sinks = [x for x in t._topo_sinks if x is not self.tsink]
 
     def __del__(self):
         try:

@@ -156,11 +156,23 @@ int crux_lik_flush(crux_lik *lik);
 
 /* Evaluate n likelihoods in one fused device pass (one P-matrix launch, one
  * traversal launch per state-count/component-count class, one reduction
- * launch).  Each lik must be on a different tree-polarity, or repeat the
- * same one with the same root.  out[i] receives lnL of liks[i] rooted at
- * roots[i].  Synchronous. */
+ * launch).  Each lik must be on a different tree-polarity.  out[i] receives
+ * lnL of liks[i] rooted at roots[i].  Synchronous.  Replaces, for a batch,
+ * the per-Lik CxLikExecute calls of Lik.lnL (CxLik.h:285, Lik.pyx:1699). */
 int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks,
                    const int32_t *roots, double *out);
+
+/* The same, split: crux_batch_submit plans the batch and enqueues its device
+ * work, returning at once with a ticket; crux_batch_wait(ticket) blocks until
+ * that batch is done and copies its n results.  Two batches can be in flight
+ * per context: the host plans and uploads the next while the device runs the
+ * current one (a third submit first completes the oldest).  Batches run in
+ * submission order, so a tree may appear in consecutive in-flight batches.
+ * Every ticket is collected once.  No counterpart in the reference: this is
+ * the asynchronous submit/poll surface of a serving loop. */
+int crux_batch_submit(crux_ctx *ctx, int n, crux_lik *const *liks,
+                      const int32_t *roots, uint64_t *ticket);
+int crux_batch_wait(crux_ctx *ctx, uint64_t ticket, int n, double *out);
 
 /* Execution switches of the context (bench and tests):
  *   mode 0: fused (default) -- one traversal launch per batch;
