@@ -3,14 +3,18 @@
 // HBM layout (all FP32 CLVs are site-major like the reference's cLMat,
 // lib/Crux/Tree/CxLik.h:10-36, but FP32 with an int32 power-of-two exponent
 // per site instead of an FP64 lnScale):
-//   CLV slot   : float  clv[S][C][K]   then   int32 exp[S]   (K != 4)
+//   CLV slot   : float  clv[S][C][KP]  then   int32 exp[S]   (K != 4; KP = K
+//                rounded up to a multiple of 4, states >= K held at 0)
 //                float  clv[C][S][4]   then   int32 exp[S]   (K == 4: component
 //                planes, so a wave's float4 access covers 1 KiB contiguously)
 //   tip codes  : uint8  codes[ntaxa][S]      index into the tree's code table
-//   tip masks  : uint8  masks[ntaxa][S64]    K == 4: 4-bit state set, rows
-//                padded to S64 = S rounded up to 64
-//   P block    : float  P[C][K][K]           row = parent state
-//   tip table  : float  T[ncodes][C][K]      T = P * leaf vector of each code
+//   tip rows   : uint8  rows[ntaxa][S64]     K == 4: 4-bit state set; K != 4:
+//                the code index; rows padded to S64 = S rounded up to 64
+//   P block    : float  P[CM][4][4]          K == 4, row = parent state
+//   P image    : float  img[NVP/4][64][4]    K != 4: P laid out as the matrix
+//                core's A operand (see img_nv)
+//   tip table  : float  T[ncodes][CM][KP]    K != 4: T = P * leaf vector of each
+//                code
 //   root weight: double piw[C][K]            pi_norm * weightScaled
 #pragma once
 
@@ -20,7 +24,6 @@
 
 namespace crux {
 
-constexpr int kTile = 256;            // sites per block in the traversal
 constexpr int kOpMaxChild = 3;        // children per fused node update
 
 enum ChildKind : int32_t {
@@ -63,9 +66,27 @@ __host__ __device__ inline int op_pos(int32_t packed, int j)
     return (int)(int8_t)((uint32_t)packed >> (8 * j));
 }
 
-// Sites per block of the 4-state traversal (one wave) and of the generic one.
+// Sites per block (one wave) of the 4-state traversal.
 constexpr int kTile4 = 64;
-__host__ __device__ inline int tile_sites(int K) { return K == 4 ? kTile4 : kTile; }
+
+// ---- 20/22-state traversal geometry -------------------------------------
+// A wave covers CM components of 64 / CM sites: lane l holds component
+// l / (64/CM) of site l % (64/CM), all KP states in its registers.  P * x
+// runs on v_mfma_f32_4x4x1f32 (16 blocks of 4 lanes, block b's output lands
+// in lanes 4b..4b+3): a lane group of one component is 16/CM blocks, and the
+// A operand of one block is broadcast to its whole group (CBSZ = log2(16/CM),
+// ABID = block), so one register carries 16/CM columns of four P rows.
+__host__ __device__ constexpr int kpad(int K) { return K == 4 ? 4 : (K + 3) & ~3; }
+__host__ __device__ constexpr int sites20(int CM) { return 64 / CM; }
+__host__ __device__ constexpr int img_cols(int CM) { return 16 / CM; }   // columns per register
+__host__ __device__ constexpr int img_groups(int KP, int CM) { return (KP + img_cols(CM) - 1) / img_cols(CM); }
+// Registers of one P image: register v = I * groups + g holds, in lane
+// c * (64/CM) + 4 a + i, P_c[4 I + i][g * cols + a] (0 outside the matrix or
+// for c >= C).
+__host__ __device__ constexpr int img_nv(int KP, int CM) { return (KP / 4) * img_groups(KP, CM); }
+__host__ __device__ constexpr int img_floats(int KP, int CM) { return ((img_nv(KP, CM) + 3) & ~3) * 64; }
+
+__host__ __device__ inline int tile_sites(int K, int CM) { return K == 4 ? kTile4 : sites20(CM); }
 
 // Largest per-lane LDS stack depth of the 4-state traversal for a component
 // class CM (keeps the stack at or below 32 KiB per block).
@@ -83,6 +104,20 @@ __host__ __device__ inline int stack_depth4(int CM, int tuned)
     return d >= 8 ? 8 : (d >= 4 ? 4 : 2);
 }
 
+// Stack depth of the 20/22-state traversal: a slot holds KP floats per lane
+// (5 or 6 KiB per wave); 4 slots unless tuned, so that about seven waves
+// share a CU's LDS.
+__host__ __device__ inline int stack_depth20(int tuned)
+{
+    const int d = tuned > 0 ? tuned : 4;
+    return d >= 8 ? 8 : (d >= 4 ? 4 : 2);
+}
+
+__host__ __device__ inline int stack_depth(int K, int CM, int tuned)
+{
+    return K == 4 ? stack_depth4(CM, tuned) : stack_depth20(tuned);
+}
+
 struct Entry {
     const Op *ops;
     int32_t nops;
@@ -96,35 +131,47 @@ struct Entry {
     double *site_out;      // [S] pattern-weighted site lnLs
     double *partials;      // [ntiles][2] compensated block sums
     double *lnl_out;       // final lnL
-    const uint8_t *const *tip_rows;  // 4-state: mask row of each tip child, plan order
+    const uint8_t *const *tip_rows;  // padded tip row of each tip child, plan order
     int32_t ntips;
     int32_t pad_;
     // 4-state: P blocks of the plan, dense [op][child][CM][16] with CM the
     // component class (rows of components >= C zero; child slots without a P
     // block are left unused), staged into LDS a chunk of updates at a time.
     const float *pdense;
-    // 4-state: control words of each update, [op][8]: nchild | kinds << 8 |
-    // from_stack << 16 | forwarded << 20 (bit j: child j is the previous
-    // update's result, kept in registers), pos, tip_load, has_parent, parent
-    // (2 words), parent_exp (2 words); staged with the P blocks.  The tip
-    // list (tip_rows, ntips) is per entry, so a one-update entry starts its
-    // own.
+    // Control words of each update, kCtlWords (4-state) or kCtlWords20 per
+    // update: nchild | kinds << 8 | from_stack << 16 | forwarded << 20 (bit
+    // j: child j is the previous update's result, kept in registers), pos,
+    // tip_load, has_parent, parent (2 words), parent_exp (2 words); for
+    // K != 4 then per child j at 8 + 6 j: P image or tip table, CLV, exponents
+    // (2 words each; the context's zero block where the child has none).  The
+    // tip list (tip_rows, ntips) is per entry, so a one-update entry starts
+    // its own.
     const int32_t *ctl;
+    // K != 4: a block of zeros that loads without a target are pointed at.
+    const float *zero;
 };
 
 constexpr int kCtlWords = 8;
+constexpr int kCtlWords20 = 32;
+constexpr size_t kZeroBytes = 64 * 1024;
+
+enum PLayout : int32_t {
+    kPBlock = 0,   // P[cm][K][K], rows of components >= C zero (4-state)
+    kPImage = 1,   // matrix-core image, img_floats(kp, cm) floats
+    kPTips = 2,    // tip table T[ncodes][cm][kp]
+};
 
 struct PJob {
     double len;            // branch length (times comp_rate[c] gives t)
     const double *model;   // per comp: V[K][K], Vinv[K][K], evals[K], rate
     const double *codevecs;// tip: [ncodes][K]
     float *out;
-    int32_t tip;
+    int32_t layout;        // PLayout
     int32_t C;
     int32_t K;
     int32_t ncodes;
-    int32_t cm;            // P blocks: rows written up to cm * K * K (zero past C)
-    int32_t pad_;
+    int32_t cm;            // component class
+    int32_t kp;            // padded state count (K != 4)
 };
 
 __host__ __device__ inline int model_stride(int K) { return 2 * K * K + K + 1; }

@@ -69,7 +69,8 @@ struct crux_ctx {
     uint64_t budget = 0;
     uint64_t used = 0;
     int mode = 0;
-    int depth4 = 0;                // 4-state LDS stack depth (0 = largest)
+    int depth4 = 0;                // traversal LDS stack depth (0 = default)
+    float *d_zero = nullptr;       // kZeroBytes of zeros (20/22-state kernel)
     bool timing = false;
     hipEvent_t ev[4] = {};
     double last_ms[4] = {0, 0, 0, 0};
@@ -203,7 +204,7 @@ struct crux_tree {
     int K = 0, S = 0, ntaxa = 0, ncodes = 0;
     uint8_t *d_codes = nullptr;
     double *d_codevecs = nullptr;
-    uint8_t *d_masks = nullptr;     // 4-state trees: 4-bit state mask per (taxon, site)
+    uint8_t *d_masks = nullptr;     // padded tip rows: 4-bit state mask (K == 4) or code
     uint32_t *d_weights = nullptr;
     bool has_topology = false;
     int nnodes = 0, nedges = 0;
@@ -287,7 +288,7 @@ static int arena_layout(crux_tree *t, int pol, int C)
         return CRUX_OK;
     arena_reset(t, pol);
     a.C = C;
-    a.clv_bytes = align_up((size_t)t->S * C * t->K * sizeof(float), 256);
+    a.clv_bytes = align_up((size_t)t->S * C * kpad(t->K) * sizeof(float), 256);
     a.slot_bytes = a.clv_bytes + align_up(t->exp_count(C) * sizeof(int32_t), 256);
     a.chunk_slots = 0;
     return CRUX_OK;
@@ -776,42 +777,40 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     const bool plain = ctx->mode == 1;
     // Sizes.
     size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0, ntips_total = 0;
+    size_t ctl_words = 0;
     size_t shmem = 0;
     for (Item &it : items) {
         crux_lik *L = it.L;
         crux_tree *t = L->tree;
         const int K = t->K, C = L->C;
-        // The 4-state kernel hands results to parents through its LDS stack;
-        // one-launch-per-update mode cannot (LDS does not outlive a launch).
-        if (K == 4 && !plain)
-            schedule_stack(L->ops, stack_depth4(it.CM, ctx->depth4));
-        if (K == 4) {
-            it.tips = assign_tips(L->ops, plain);
-            ntips_total += it.tips.size();
-        }
+        // The traversal kernels hand results to parents through their LDS
+        // stack; one-launch-per-update mode cannot (LDS does not outlive a
+        // launch).
+        if (!plain)
+            schedule_stack(L->ops, stack_depth(K, it.CM, ctx->depth4));
+        it.tips = assign_tips(L->ops, plain);
+        ntips_total += it.tips.size();
         nops += L->ops.size();
+        ctl_words += L->ops.size() * (K == 4 ? kCtlWords : kCtlWords20);
         if (plain)
             nplain += L->ops.size();
         if (K == 4)
             pm_floats += align_up(L->ops.size() * kOpMaxChild * it.CM * 16, 64);
         for (PlanOp &op : L->ops)
             for (PlanChild &c : op.ch) {
+                if (c.kind == kChildIdent)
+                    continue;
+                njobs++;
                 if (K == 4) {
-                    if (c.kind != kChildIdent) {
-                        njobs++;
-                        shmem = std::max(shmem, (size_t)C * K * sizeof(double));
-                    }
-                } else if (c.kind == kChildInternal) {
-                    njobs++;
-                    pm_floats += align_up((size_t)C * K * K, 64);
                     shmem = std::max(shmem, (size_t)C * K * sizeof(double));
-                } else if (c.kind == kChildTip) {
-                    njobs++;
-                    pm_floats += align_up((size_t)t->ncodes * C * K, 64);
+                } else {
+                    pm_floats += align_up(c.kind == kChildTip ? (size_t)t->ncodes * it.CM * kpad(K)
+                                                              : (size_t)img_floats(kpad(K), it.CM),
+                                          64);
                     shmem = std::max(shmem, ((size_t)C * K + (size_t)C * K * K) * sizeof(double));
                 }
             }
-        tiles += (t->S + tile_sites(t->K) - 1) / tile_sites(t->K);
+        tiles += (t->S + tile_sites(K, it.CM) - 1) / tile_sites(K, it.CM);
     }
     const size_t nent = items.size() + nplain;
     const size_t off_ent = 0;
@@ -821,13 +820,14 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     const size_t nblk_total = plain ? tiles * 0 : tiles;
     size_t blk_count = 0;
     for (Item &it : items) {
-        size_t nt = (it.L->tree->S + tile_sites(it.L->tree->K) - 1) / tile_sites(it.L->tree->K);
+        const size_t ts = tile_sites(it.L->tree->K, it.CM);
+        size_t nt = (it.L->tree->S + ts - 1) / ts;
         blk_count += plain ? nt * (it.L->ops.size() + 1) : nt;
     }
     (void)nblk_total;
     const size_t off_tips = align_up(off_blk + blk_count * sizeof(int32_t), 256);
     const size_t off_ctl = align_up(off_tips + ntips_total * sizeof(void *), 256);
-    const size_t plan_bytes = align_up(off_ctl + nops * kCtlWords * sizeof(int32_t), 256);
+    const size_t plan_bytes = align_up(off_ctl + ctl_words * sizeof(int32_t), 256);
     if (shmem > 64 * 1024)
         return set_error(CRUX_EINVAL, "P-matrix job too large for LDS");
 
@@ -876,7 +876,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     const PJob *djobs = (const PJob *)(D + off_jobs);
     const int32_t *dblk = (const int32_t *)(D + off_blk);
 
-    size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0, itip = 0;
+    size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0, itip = 0, ictl = 0;
     uint64_t traffic = 0, formula = 0;
     int nodes = 0;
     // Fused launches: groups of consecutive items of one (K, CM) class.
@@ -892,12 +892,13 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         crux_tree *t = L->tree;
         const int K = t->K, C = L->C;
         const Arena &A = t->arena[L->pol];
-        const size_t ntl = (t->S + tile_sites(t->K) - 1) / tile_sites(t->K);
-        const size_t op0 = io;
-        const uint64_t clv_b = (uint64_t)t->S * C * K * 4, exp_b = (uint64_t)t->exp_count(C) * 4;
+        const int CM = items[ii].CM, KP = kpad(K);
+        const int nw = K == 4 ? kCtlWords : kCtlWords20;
+        const size_t ntl = (t->S + tile_sites(K, CM) - 1) / tile_sites(K, CM);
+        const size_t op0 = io, ctl0 = ictl;
+        const uint64_t clv_b = (uint64_t)t->S * C * KP * 4, exp_b = (uint64_t)t->exp_count(C) * 4;
         // 4-state: dense [op][child][CM][16] P region of this evaluation.
         const size_t pdense = ipm;
-        const int CM = items[ii].CM;
         if (K == 4)
             ipm += align_up(L->ops.size() * kOpMaxChild * CM * 16, 64);
         for (PlanOp &pop : L->ops) {
@@ -911,8 +912,15 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                 packed |= (uint32_t)(uint8_t)pop.pos[j] << (8 * j);
             op.pos = (int32_t)packed;
             op.tip_load = pop.tip_load;
+            int32_t *cw = (int32_t *)(H + off_ctl) + ictl;
+            ictl += nw;
+            auto put_ptr = [&](int k, const void *p) {
+                const uint64_t v = (uint64_t)(uintptr_t)p;
+                cw[k] = (int32_t)(uint32_t)v;
+                cw[k + 1] = (int32_t)(uint32_t)(v >> 32);
+            };
             auto write_ctl = [&]() {
-                int32_t *cw = (int32_t *)(H + off_ctl) + (io - 1) * kCtlWords;
+                memset(cw, 0, nw * sizeof(int32_t));
                 cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20);
                 cw[1] = op.pos;
                 cw[2] = op.tip_load;
@@ -923,6 +931,17 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                 cw[5] = (int32_t)(uint32_t)(pp >> 32);
                 cw[6] = (int32_t)(uint32_t)pe;
                 cw[7] = (int32_t)(uint32_t)(pe >> 32);
+                if (K == 4)
+                    return;
+                // Per child: P image or tip table, CLV, exponents; the zero
+                // block where the child has none.
+                for (int j = 0; j < kOpMaxChild; j++) {
+                    const OpChild &oc = op.ch[j];
+                    const bool have = j < op.nchild;
+                    put_ptr(8 + 6 * j, have && oc.mat ? (const void *)oc.mat : ctx->d_zero);
+                    put_ptr(10 + 6 * j, have && oc.clv ? (const void *)oc.clv : ctx->d_zero);
+                    put_ptr(12 + 6 * j, have && oc.exps ? (const void *)oc.exps : ctx->d_zero);
+                }
             };
             if (pop.parent_slot >= 0) {
                 op.parent = A.clv(pop.parent_slot);
@@ -957,16 +976,19 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                     jb.model = L->d_model;
                     jb.codevecs = t->d_codevecs;
                     // The 4-state kernel forms tip products from P itself.
-                    jb.tip = pc.kind == kChildTip && K != 4;
+                    jb.layout = K == 4 ? kPBlock : pc.kind == kChildTip ? kPTips : kPImage;
                     jb.C = C;
                     jb.K = K;
                     jb.ncodes = t->ncodes;
-                    jb.cm = K == 4 && !jb.tip ? CM : C;
+                    jb.cm = CM;
+                    jb.kp = KP;
                     if (K == 4) {
                         jb.out = S.d_pmat + pdense + (olocal * kOpMaxChild + j) * CM * 16;
                     } else {
                         jb.out = S.d_pmat + ipm;
-                        ipm += align_up(jb.tip ? (size_t)t->ncodes * C * K : (size_t)C * K * K, 64);
+                        ipm += align_up(jb.layout == kPTips ? (size_t)t->ncodes * CM * KP
+                                                            : (size_t)img_floats(KP, CM),
+                                        64);
                     }
                     oc.mat = jb.out;
                 }
@@ -975,7 +997,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         }
         Entry base{};
         base.ops = dops + op0;
-        base.ctl = (const int32_t *)(D + off_ctl) + op0 * kCtlWords;
+        base.ctl = (const int32_t *)(D + off_ctl) + ctl0;
+        base.zero = ctx->d_zero;
         base.nops = (int32_t)L->ops.size();
         base.S = t->S;
         base.C = C;
@@ -1016,7 +1039,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                 e1.nops = 1;
                 if (base.pdense)
                     e1.pdense = base.pdense + o * kOpMaxChild * CM * 16;
-                e1.ctl = base.ctl + o * kCtlWords;
+                e1.ctl = base.ctl + o * nw;
                 if (base.tip_rows) {
                     e1.tip_rows = base.tip_rows + L->ops[o].tip_first;
                     e1.ntips = L->ops[o].tip_count;
@@ -1045,7 +1068,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         if (ln.nblk == 0)
             continue;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
-                            stack_depth4(ln.CM, ctx->depth4), st) != 0)
+                            stack_depth(ln.K, ln.CM, ctx->depth4), st) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
         HIP_TRY(hipGetLastError());
     }
@@ -1159,7 +1182,13 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
         e = hipEventCreate(&ctx->ev[i]);
     for (int k = 0; k < 2 && e == hipSuccess; k++)
         e = hipEventCreateWithFlags(&ctx->stg[k].done, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&ctx->d_zero, kZeroBytes);
+    if (e == hipSuccess)
+        e = hipMemset(ctx->d_zero, 0, kZeroBytes);
     if (e != hipSuccess) {
+        if (ctx->d_zero)
+            (void)hipFree(ctx->d_zero);
         delete ctx;
         return set_error(CRUX_EDEVICE, "context setup: %s", hipGetErrorString(e));
     }
@@ -1194,6 +1223,7 @@ int crux_ctx_destroy(crux_ctx *ctx)
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i])
             (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipFree(ctx->d_zero);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CRUX_OK;
@@ -1301,16 +1331,17 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
                       hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(t->d_weights, weights, sizeof(uint32_t) * nsites, hipMemcpyHostToDevice);
-    if (e == hipSuccess && dim == 4) {
-        // The 4-state kernel reads the leaf vector itself as a 4-bit mask;
-        // rows are padded to whole 64-site tiles (read four sites per lane).
+    if (e == hipSuccess) {
+        // Tip rows padded to whole 64-site tiles (the kernels read four sites
+        // per lane).  The 4-state kernel reads the leaf vector itself as a
+        // 4-bit mask; the 20/22-state kernel the code, for its tip table.
         std::vector<uint8_t> mask_of(ncodes), masks((size_t)ntaxa * t->mask_stride(), 0);
         for (int c = 0; c < ncodes; c++) {
             uint8_t m = 0;
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < 4 && dim == 4; j++)
                 if (code_vecs[(size_t)c * 4 + j] != 0.0)
                     m |= (uint8_t)(1u << j);
-            mask_of[c] = m;
+            mask_of[c] = dim == 4 ? m : (uint8_t)c;
         }
         for (int x = 0; x < ntaxa; x++)
             for (int s = 0; s < nsites; s++)

@@ -5,26 +5,26 @@
 //              leaf vector of every distinct code instead (tip table).  This
 //              replaces the per-step, per-stripe CxLikPt calls of the
 //              reference (lib/Crux/Tree/CxLik.c:334-347).
-//  k_traverse  The CLV updates of a whole plan.  One lane owns one site
-//              across all components; a block owns 256 consecutive sites of
-//              one evaluation and walks every node update of the plan in
-//              post-order, so a lane only ever reads CLV entries it wrote
-//              itself in an earlier update: one launch covers the whole
-//              traversal and every evaluation of a batch with no barrier.
-//              Node update = product over children of (P * child) (or a tip
-//              table lookup), then a power-of-two rescale of the site so its
-//              largest entry is in [0.5, 1), the exponent added to the
-//              children's.  This is the arithmetic of CxLikExecuteStripe
+//  k_trav4     The CLV updates of a whole 4-state plan: one wave per 64
+//  k_trav20    sites (k_trav4) or per 64 (site, component) lanes (k_trav20,
+//              20 and 22 states) walks every node update of the plan in
+//              post-order, so one launch covers the whole traversal and
+//              every evaluation of a batch with no barrier.  Node update =
+//              product over children of (P * child) (or a tip lookup), then a
+//              power-of-two rescale of the site so its largest entry is in
+//              [0.5, 1), the exponent added to the children's.  This is the
+//              arithmetic of CxLikExecuteStripeDna / CxLikExecuteStripe
 //              (CxLik.c:294-846) with FP32 storage and exact power-of-two
-//              scaling in place of FP64 division by the max.  The root
-//              update is not stored: it is reduced to the pattern-weighted
-//              site lnL in FP64, NaN -> -inf (CxLik.c:598-632), and to a
-//              compensated per-block sum.
+//              scaling in place of FP64 division by the max.  The root update
+//              is not stored: it is reduced to the pattern-weighted site lnL
+//              in FP64, NaN -> -inf (CxLik.c:598-632), and to a compensated
+//              per-wave sum.
 //  k_finalize  Compensated FP64 sum of the per-block partials per evaluation.
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "device.h"
 
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
     const PJob J = jobs[blockIdx.x];
     const int C = J.C, K = J.K, KK = K * K, st = model_stride(K);
     double *ex = sm;            // [C][K] exp(lambda_k * t_c)
-    double *P = sm + C * K;     // [C][K][K], tip jobs only
+    double *P = sm + C * K;     // [C][K][K], images and tip tables
     for (int idx = threadIdx.x; idx < C * K; idx += blockDim.x) {
         const int c = idx / K, k = idx - c * K;
         const double *m = J.model + (size_t)c * st;
@@ -128,209 +128,62 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
         for (int k = 0; k < K; ++k)
             p += V[i * K + k] * e[k] * Vi[k * K + j];
         p = p < 0.0 ? 0.0 : p;
-        if (J.tip)
-            P[idx] = p;
-        else
+        if (J.layout == kPBlock)
             J.out[idx] = (float)p;
+        else
+            P[idx] = p;
     }
-    // 4-state P blocks are strided by the component class: the rows of the
-    // idle components c in [C, cm) are zero.
-    for (int idx = C * KK + (int)threadIdx.x; idx < J.cm * KK; idx += blockDim.x)
-        J.out[idx] = 0.0f;
-    if (!J.tip)
+    if (J.layout == kPBlock) {
+        // 4-state P blocks are strided by the component class: the rows of
+        // the idle components c in [C, cm) are zero.
+        for (int idx = C * KK + (int)threadIdx.x; idx < J.cm * KK; idx += blockDim.x)
+            J.out[idx] = 0.0f;
         return;
+    }
     __syncthreads();
-    const int n = J.ncodes * C * K;
+    const int cm = J.cm, kp = J.kp;
+    if (J.layout == kPImage) {
+        // Register v of lane l (float v % 4 of float4 v / 4 of the lane):
+        // P_c[4 I + i][g * cols + a] for v = I * groups + g and
+        // l = c * (64 / cm) + 4 a + i.
+        const int G = 64 / cm, W = 16 / cm, NG = (kp + W - 1) / W, NV = (kp / 4) * NG;
+        const int n = ((NV + 3) & ~3) * 64;
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            const int v = (idx >> 8) * 4 + (idx & 3), l = (idx >> 2) & 63;
+            float val = 0.0f;
+            if (v < NV) {
+                const int I = v / NG, g = v - I * NG, c = l / G, a = (l % G) >> 2;
+                const int row = 4 * I + (l & 3), col = g * W + a;
+                if (c < C && row < K && col < K)
+                    val = (float)P[(c * K + row) * K + col];
+            }
+            J.out[idx] = val;
+        }
+        return;
+    }
+    // Tip table: T[code][c][i] = sum_j P_c[i][j] leaf[code][j].
+    const int n = J.ncodes * cm * kp;
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        const int code = idx / (C * K), r = idx - code * C * K;   // r = c*K + i
-        const double *row = P + (size_t)r * K, *v = J.codevecs + (size_t)code * K;
+        const int code = idx / (cm * kp), r = idx - code * cm * kp, c = r / kp, i = r - c * kp;
         double t = 0.0;
-        for (int j = 0; j < K; ++j)
-            t += row[j] * v[j];
+        if (c < C && i < K) {
+            const double *row = P + (size_t)(c * K + i) * K, *v = J.codevecs + (size_t)code * K;
+            for (int j = 0; j < K; ++j)
+                t += row[j] * v[j];
+        }
         J.out[idx] = (float)t;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Traversal.
+// Traversal helpers.
 
-template <int K>
-__device__ __forceinline__ void load_vec(const CRUX_GAS float *__restrict__ p, float (&x)[K])
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F &&f)
 {
-    if constexpr (K % 4 == 0) {
-#pragma unroll
-        for (int q = 0; q < K / 4; ++q) {
-            const float4 v = ld4(p + 4 * q);
-            x[4 * q + 0] = v.x;
-            x[4 * q + 1] = v.y;
-            x[4 * q + 2] = v.z;
-            x[4 * q + 3] = v.w;
-        }
-    } else if constexpr (K % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < K / 2; ++q) {
-            const float2 v = ld2(p + 2 * q);
-            x[2 * q + 0] = v.x;
-            x[2 * q + 1] = v.y;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < K; ++q)
-            x[q] = p[q];
-    }
-}
-
-template <int K>
-__device__ __forceinline__ void store_vec(CRUX_GAS float *__restrict__ p, const float (&x)[K])
-{
-    if constexpr (K % 4 == 0) {
-#pragma unroll
-        for (int q = 0; q < K / 4; ++q)
-            st4(p + 4 * q, make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]));
-    } else if constexpr (K % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < K / 2; ++q)
-            st2(p + 2 * q, make_float2(x[2 * q], x[2 * q + 1]));
-    } else {
-#pragma unroll
-        for (int q = 0; q < K; ++q)
-            p[q] = x[q];
-    }
-}
-
-template <int K, int CM>
-__global__ void __launch_bounds__(kTile)
-k_traverse(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
-{
-    __shared__ double red_s[kTile];
-    __shared__ double red_c[kTile];
-    // block_entry is indexed by the block's position among all blocks of the
-    // batch; this launch covers positions [blk0, blk0 + gridDim.x).
-    const int b = blk0 + (int)blockIdx.x;
-    const int e = cas(block_entry)[b];
-    const CRUX_CAS Entry *E = cas(entries) + e;
-    const int S = E->S, C = E->C, nops = E->nops;
-    const int tile = b - E->tile0;
-    const int s = tile * kTile + (int)threadIdx.x;
-    const int CK = C * K;
-    double sum = 0.0, cmp = 0.0;
-
-    if (s < S) {
-        for (int o = 0; o < nops; ++o) {
-            const CRUX_CAS Op *op = cas(E->ops) + o;
-            const int nch = op->nchild, kinds = op->kinds;
-            float acc[CM][K];
-            int32_t ex = 0;
-#pragma unroll
-            for (int j = 0; j < kOpMaxChild; ++j) {
-                if (j >= nch)
-                    break;
-                const CRUX_CAS OpChild &ch = op->ch[j];
-                const int kind = (kinds >> (2 * j)) & 3;
-                if (kind == kChildTip) {
-                    const int code = gas(ch.codes)[s];
-                    const CRUX_GAS float *tb = gas(ch.mat) + (size_t)code * CK;
-#pragma unroll
-                    for (int c = 0; c < CM; ++c) {
-                        if (c < C) {
-                            float x[K];
-                            load_vec<K>(tb + c * K, x);
-#pragma unroll
-                            for (int i = 0; i < K; ++i)
-                                acc[c][i] = (j == 0) ? x[i] : acc[c][i] * x[i];
-                        }
-                    }
-                } else {
-                    const CRUX_GAS float *src = gas(ch.clv) + (size_t)s * CK;
-                    ex += gas(ch.exps)[s];
-                    if (kind == kChildIdent) {
-#pragma unroll
-                        for (int c = 0; c < CM; ++c) {
-                            if (c < C) {
-                                float x[K];
-                                load_vec<K>(src + c * K, x);
-#pragma unroll
-                                for (int i = 0; i < K; ++i)
-                                    acc[c][i] = (j == 0) ? x[i] : acc[c][i] * x[i];
-                            }
-                        }
-                    } else {
-                        const CRUX_CAS float *__restrict__ P = cas(ch.mat);
-#pragma unroll
-                        for (int c = 0; c < CM; ++c) {
-                            if (c < C) {
-                                float x[K];
-                                load_vec<K>(src + c * K, x);
-#pragma unroll
-                                for (int i = 0; i < K; ++i) {
-                                    const CRUX_CAS float *Pr = P + (c * K + i) * K;
-                                    float v = 0.0f;
-#pragma unroll
-                                    for (int q = 0; q < K; ++q)
-                                        v = fmaf(Pr[q], x[q], v);
-                                    acc[c][i] = (j == 0) ? v : acc[c][i] * v;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            if (op->parent != nullptr) {
-                float m = 0.0f;
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    if (c < C)
-#pragma unroll
-                        for (int i = 0; i < K; ++i)
-                            m = fmaxf(m, acc[c][i]);
-                int e2 = 0;
-                if (m > 0.0f)
-                    (void)frexpf(m, &e2);
-                CRUX_GAS float *dst = gas(op->parent) + (size_t)s * CK;
-#pragma unroll
-                for (int c = 0; c < CM; ++c) {
-                    if (c < C) {
-                        float y[K];
-#pragma unroll
-                        for (int i = 0; i < K; ++i)
-                            y[i] = ldexpf(acc[c][i], -e2);
-                        store_vec<K>(dst + c * K, y);
-                    }
-                }
-                gas(op->parent_exp)[s] = ex + e2;
-            } else {
-                const CRUX_CAS double *piw = cas(E->piw);
-                double L = 0.0;
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    if (c < C)
-#pragma unroll
-                        for (int i = 0; i < K; ++i)
-                            L += piw[c * K + i] * (double)acc[c][i];
-                double v = (log(L) + (double)ex * 0.69314718055994530942) * (double)gas(E->weights)[s];
-                if (__builtin_isnan(v))
-                    v = -INFINITY;
-                gas(E->site_out)[s] = v;
-                sum = v;
-            }
-        }
-    }
-    red_s[threadIdx.x] = sum;
-    red_c[threadIdx.x] = cmp;
-    __syncthreads();
-#pragma unroll
-    for (int w = kTile / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            double a = red_s[threadIdx.x], b = red_c[threadIdx.x];
-            nsum_merge(a, b, red_s[threadIdx.x + w], red_c[threadIdx.x + w]);
-            red_s[threadIdx.x] = a;
-            red_c[threadIdx.x] = b;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        gas(E->partials)[2 * tile] = red_s[0];
-        gas(E->partials)[2 * tile + 1] = red_c[0];
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
     }
 }
 
@@ -719,6 +572,331 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
     }
 }
 
+// ---------------------------------------------------------------------------
+// 20/22-state traversal.  A block is one wave covering CM mixture components
+// of G = 64 / CM consecutive sites: lane l holds component l / G of site
+// l % G, all KP states of it in registers (states >= K stay 0).  Like the
+// 4-state kernel, the wave walks the whole plan for its sites with no
+// barrier, results of an update are stored to their HBM slot ([S][C][KP])
+// and handed to the consumer in registers (the update just before it) or
+// through the wave's LDS stack, and the root update reduces to the
+// pattern-weighted FP64 site lnL (NaN -> -inf, CxLik.c:598-632) and a
+// compensated sum per wave.  Arithmetic follows the generic stripe
+// CxLikExecuteStripe (CxLik.c:635-846) with FP32 storage and power-of-two
+// scaling.
+//
+// An internal child is y = P_c x on the matrix core: v_mfma_f32_4x4x1f32
+// with lane l supplying B = its own x[col] and, broadcast from block ABID of
+// its component's lane group, A = P_c[4 I + i][col] for the group's four
+// rows i.  One MFMA per (row block I, column) gives each lane y[4 I .. 4 I +
+// 3] of its own (site, component); the sum is the in-order FMA chain over
+// the columns.  The A operands come from a P image laid out per lane by
+// k_pmat, one float per register, so an update's P for all components is a
+// few coalesced float4 loads per lane.  A tip child is a row of the tip
+// table (P_c times the leaf vector of the site's code; the codes of up to
+// kTipChunk tips are staged in LDS).  Everything a child needs from memory
+// -- P image or tip row, and the CLV of a child cached by an earlier plan --
+// is loaded one update ahead: right after child j of update o is consumed,
+// the loads for child j of update o + 1 are issued into the same registers,
+// so they are in flight while the rest of update o runs.  The control words
+// of an update are loaded two updates ahead (one dword per lane, read out
+// with readlane).  A third child (the root update and chained
+// multifurcations) is loaded where it is used.
+template <int KP, int CM, int D>
+__global__ void __launch_bounds__(64)
+k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
+{
+    constexpr int G = sites20(CM);                  // sites per wave
+    constexpr int W = img_cols(CM);                 // P columns per image register
+    constexpr int CBSZ = W == 16 ? 4 : W == 8 ? 3 : W == 4 ? 2 : W == 2 ? 1 : 0;
+    constexpr int NI = KP / 4;                      // row blocks = float4 per CLV row
+    constexpr int NG = img_groups(KP, CM);
+    constexpr int NQ = img_floats(KP, CM) / 256;    // float4 per lane of a P image
+    constexpr int PQ = NQ > NI ? NQ : NI;           // float4 of a prefetched child
+    constexpr int WPR = G / 4;                      // code dwords per staged tip row
+    constexpr int RPI = 64 / WPR;                   // tip rows per staging load
+    static_assert((1 << CBSZ) == W, "component lane groups are whole matrix blocks");
+    __shared__ crux_f32x4 stk[D][NI][64];
+    __shared__ int32_t stke[D][64];
+    __shared__ uint32_t tipw[kTipChunk][WPR];
+
+    const int b = blk0 + (int)blockIdx.x;
+    const CRUX_CAS Entry *E = cas(entries) + cas(block_entry)[b];
+    const int S = E->S, C = E->C, K = E->K, nops = E->nops, ntips = E->ntips;
+    const int tile = b - E->tile0;
+    const int lane = (int)threadIdx.x;
+    const int c = lane / G, t = lane % G;
+    const int s = tile * G + t;
+    const bool site_ok = s < S;
+    const int sc = site_ok ? s : S - 1;     // in-bounds site for loads of idle lanes
+    const int cc = c < C ? c : C - 1;       // in-bounds component for idle lanes
+    const CRUX_GAS int32_t *ctl = gas(E->ctl);
+    const char *zero = reinterpret_cast<const char *>(E->zero);
+    double sum = 0.0;
+
+    auto word = [](int32_t cw, int k) -> int32_t { return __builtin_amdgcn_readlane(cw, k); };
+    auto ptr = [&](int32_t cw, int k) -> const char * {
+        const uint64_t lo = (uint32_t)word(cw, k), hi = (uint32_t)word(cw, k + 1);
+        return reinterpret_cast<const char *>(lo | (hi << 32));
+    };
+    auto ctl_of = [&](int o) -> int32_t { return ctl[o * kCtlWords20 + (lane & 31)]; };
+
+    // Codes of tips [base, base + kTipChunk) of the entry's list, for this
+    // wave's sites (rows are padded to whole 64-site tiles).
+    auto stage_tips = [&](int base) {
+        const int n = min(kTipChunk, ntips - base);
+        const int sub = lane / WPR, w = lane % WPR;
+        uint32_t v[kTipChunk / RPI];
+#pragma unroll
+        for (int i = 0; i < kTipChunk / RPI; ++i) {
+            const int r = i * RPI + sub;
+            v[i] = 0u;
+            if (r < n) {
+                const uint8_t *row = gas(E->tip_rows)[base + r];
+                v[i] = *(const CRUX_GAS uint32_t *)gas(row + tile * G + 4 * w);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kTipChunk / RPI; ++i) {
+            const int r = i * RPI + sub;
+            if (r < n)
+                tipw[r][w] = v[i];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    };
+
+    // Loads for child j of the update with control words cw (ok: the update
+    // exists).  pr: the P image (internal child) or the tip-table row (tip);
+    // hx/he: the child's CLV and exponent when it comes from HBM.  Every load
+    // is issued whatever the child is -- missing targets read the zero
+    // block -- so no register is merged across branches.
+    auto fetch = [&](int32_t cw, bool ok, int j, crux_f32x4 (&pr)[PQ], crux_f32x4 (&hxr)[NI],
+                     int32_t &her) {
+        const int w0 = ok ? word(cw, 0) : 0;
+        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, via = ((w0 >> 16) | (w0 >> 20)) & 0xf;
+        const bool have = j < nch;
+        const int kind = have ? (kinds >> (2 * j)) & 3 : (int)kChildIdent;
+        const bool hbm = have && kind != kChildTip && !((via >> j) & 1);
+        const bool tip = kind == kChildTip;
+        const char *mat = have && kind != kChildIdent ? ptr(cw, 8 + 6 * j) : zero;
+        const char *clv = hbm ? ptr(cw, 10 + 6 * j) : zero;
+        const char *exs = hbm ? ptr(cw, 12 + 6 * j) : zero;
+        const int pos = op_pos(ok ? word(cw, 1) : 0, j) & (kTipChunk - 1);
+        const uint32_t code = (tipw[pos][t >> 2] >> (8 * (t & 3))) & 0xffu;
+#pragma unroll
+        for (int q = 0; q < PQ; ++q) {
+            const size_t off = tip ? ((size_t)(code * CM + c) * KP + 4 * (q < NI ? q : 0)) * 4
+                                   : (size_t)((q < NQ ? q : 0) * 64 + lane) * 16;
+            pr[q] = *(const CRUX_GAS crux_f32x4 *)gas(mat + off);
+        }
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const size_t off = hbm ? ((size_t)(sc * C + cc) * KP + 4 * q) * 4 : (size_t)q * 16;
+            hxr[q] = *(const CRUX_GAS crux_f32x4 *)gas(clv + off);
+        }
+        her = *(const CRUX_GAS int32_t *)gas(exs + (hbm ? (size_t)sc * 4 : 0));
+    };
+
+    // y = P_c x for every lane's own (site, component).
+    auto pmul = [&](const crux_f32x4 (&img)[PQ], const crux_f32x4 (&x)[NI], crux_f32x4 (&y)[NI]) {
+#pragma unroll
+        for (int I = 0; I < NI; ++I)
+            y[I] = crux_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            static_for<W>([&](auto A) {
+                constexpr int a = decltype(A)::value;
+                const int col = g * W + a;
+                if (col < KP) {
+#pragma unroll
+                    for (int I = 0; I < NI; ++I) {
+                        const int v = I * NG + g;
+                        y[I] = __builtin_amdgcn_mfma_f32_4x4x1f32(img[v >> 2][v & 3], x[col >> 2][col & 3],
+                                                                  y[I], CBSZ, a, 0);
+                    }
+                }
+            });
+        }
+    };
+
+    crux_f32x4 pre[2][PQ];
+    crux_f32x4 hx[2][NI];
+    int32_t he[2];
+    int32_t cw0 = ctl_of(0), cw1 = ctl_of(min(1, nops - 1));
+    if (ntips > 0)
+        stage_tips(0);
+    fetch(cw0, true, 0, pre[0], hx[0], he[0]);
+    fetch(cw0, true, 1, pre[1], hx[1], he[1]);
+
+    // The previous update's scaled result and exponent (forwarded children).
+    crux_f32x4 yl[NI];
+    int32_t el = 0;
+#pragma unroll
+    for (int I = 0; I < NI; ++I)
+        yl[I] = crux_f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int o = 0; o < nops; ++o) {
+        const bool nxt = o + 1 < nops;
+        const int32_t cw2 = ctl_of(min(o + 2, nops - 1));
+        const int w0 = word(cw0, 0);
+        const int32_t pk = word(cw0, 1);
+        const bool has_parent = word(cw0, 3) != 0;
+        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xf,
+                  fwd = (w0 >> 20) & 0xf;
+        // The next update's tips may start a new chunk; this update's tip
+        // rows are already in registers.
+        if (nxt) {
+            const int tl = word(cw1, 2);
+            if (tl > 0)
+                stage_tips(tl);
+        }
+        crux_f32x4 acc[NI];
+        int32_t ex = 0;
+        static_for<kOpMaxChild>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            if (j < nch) {
+                const int kind = (kinds >> (2 * j)) & 3;
+                crux_f32x4 p2[j < 2 ? 1 : PQ], h2[j < 2 ? 1 : NI];
+                int32_t e2x = 0;
+                if constexpr (j == 2)
+                    fetch(cw0, true, 2, p2, h2, e2x);
+                auto &pj = [&]() -> auto & {
+                    if constexpr (j < 2)
+                        return pre[j];
+                    else
+                        return p2;
+                }();
+                auto &hj = [&]() -> auto & {
+                    if constexpr (j < 2)
+                        return hx[j];
+                    else
+                        return h2;
+                }();
+                const int32_t hej = j < 2 ? he[j < 2 ? j : 0] : e2x;
+                crux_f32x4 y[NI];
+                if (kind == kChildTip) {
+#pragma unroll
+                    for (int I = 0; I < NI; ++I)
+                        y[I] = pj[I];
+                } else {
+                    crux_f32x4 x[NI];
+                    if ((fwd >> j) & 1) {
+#pragma unroll
+                        for (int I = 0; I < NI; ++I)
+                            x[I] = yl[I];
+                        ex += el;
+                    } else if ((fst >> j) & 1) {
+                        const int p = op_pos(pk, j) & (D - 1);
+#pragma unroll
+                        for (int I = 0; I < NI; ++I)
+                            x[I] = stk[p][I][lane];
+                        ex += stke[p][lane];
+                    } else {
+#pragma unroll
+                        for (int I = 0; I < NI; ++I)
+                            x[I] = hj[I];
+                        ex += hej;
+                    }
+                    if (kind == kChildIdent) {
+                        // Used as is; idle components read a real one's CLV.
+#pragma unroll
+                        for (int I = 0; I < NI; ++I)
+                            y[I] = c < C ? x[I] : crux_f32x4{0.f, 0.f, 0.f, 0.f};
+                    } else {
+                        pmul(pj, x, y);
+                    }
+                }
+#pragma unroll
+                for (int I = 0; I < NI; ++I)
+                    acc[I] = j == 0 ? y[I] : acc[I] * y[I];
+            }
+            // Child j's registers are free: load child j of the next update.
+            if constexpr (j < 2)
+                fetch(cw1, nxt, j, pre[j], hx[j], he[j]);
+        });
+
+        if (has_parent) {
+            // Site maximum over the lane's states, then over the site's
+            // components (lanes G apart); all values are >= 0, so the
+            // largest signed integer is the largest float.
+            int32_t mb = 0;
+#pragma unroll
+            for (int I = 0; I < NI; ++I)
+                mb = max(max(mb, max(__float_as_int(acc[I].x), __float_as_int(acc[I].y))),
+                         max(__float_as_int(acc[I].z), __float_as_int(acc[I].w)));
+#pragma unroll
+            for (int off = G; off < 64; off <<= 1)
+                mb = max(mb, __shfl_xor(mb, off, 64));
+            const float m = __int_as_float(mb);
+            int e2 = 0;
+            if (m > 0.0f)
+                (void)frexpf(m, &e2);
+            el = ex + e2;
+#pragma unroll
+            for (int I = 0; I < NI; ++I)
+                yl[I] = crux_f32x4{ldexpf(acc[I].x, -e2), ldexpf(acc[I].y, -e2), ldexpf(acc[I].z, -e2),
+                                   ldexpf(acc[I].w, -e2)};
+            const int pp = op_pos(pk, kOpMaxChild);
+            if (pp >= 0) {
+#pragma unroll
+                for (int I = 0; I < NI; ++I)
+                    stk[pp][I][lane] = yl[I];
+                stke[pp][lane] = el;
+            }
+            if (site_ok && c < C) {
+                CRUX_GAS crux_f32x4 *dst = (CRUX_GAS crux_f32x4 *)gas(ptr(cw0, 4)) +
+                                           ((size_t)s * C + c) * NI;
+#pragma unroll
+                for (int I = 0; I < NI; ++I)
+                    dst[I] = yl[I];
+                if (c == 0)
+                    ((CRUX_GAS int32_t *)gas(ptr(cw0, 6)))[s] = el;
+            }
+        } else {
+            double L = 0.0;
+            if (c < C) {
+                const CRUX_GAS double *pw = gas(E->piw) + (size_t)c * K;
+#pragma unroll
+                for (int i = 0; i < KP; ++i)
+                    if (i < K)
+                        L += pw[i] * (double)acc[i >> 2][i & 3];
+            }
+            // Sum over the site's components; every lane ends with the same
+            // value (each pairwise sum is commutative).
+#pragma unroll
+            for (int off = G; off < 64; off <<= 1)
+                L += __shfl_xor(L, off, 64);
+            double v = (log(L) + (double)ex * 0.69314718055994530942) *
+                       (double)gas(E->weights)[sc];
+            if (__builtin_isnan(v))
+                v = -INFINITY;
+            if (site_ok && c == 0) {
+                gas(E->site_out)[s] = v;
+                sum = v;
+            }
+        }
+        cw0 = cw1;
+        cw1 = cw2;
+    }
+    double cmp = 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double s2 = __shfl_xor(sum, off, 64), c2 = __shfl_xor(cmp, off, 64);
+        if ((lane & off) == 0) {
+            nsum_merge(sum, cmp, s2, c2);
+        } else {
+            double a = s2, bb = c2;
+            nsum_merge(a, bb, sum, cmp);
+            sum = a;
+            cmp = bb;
+        }
+    }
+    if (lane == 0) {
+        gas(E->partials)[2 * tile] = sum;
+        gas(E->partials)[2 * tile + 1] = cmp;
+    }
+}
+
 __global__ void __launch_bounds__(64) k_finalize(const Entry *__restrict__ entries)
 {
     const Entry *E = &entries[blockIdx.x];
@@ -755,13 +933,21 @@ static void launch_trav4(const Entry *entries, const int32_t *block_entry, int b
                            entries, block_entry, blk0);
 }
 
+template <int KP, int CM, int D>
+static void launch_trav20(const Entry *entries, const int32_t *block_entry, int blk0,
+                          int nblocks, hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_trav20<KP, CM, D>), dim3(nblocks), dim3(64), 0, stream, entries,
+                       block_entry, blk0);
+}
+
 template <int K, int CM>
 static void launch_traverse(const Entry *entries, const int32_t *block_entry, int blk0,
                             int nblocks, int depth, hipStream_t stream)
 {
+    // Stack depth: more slots keep more results in LDS, fewer let more
+    // blocks share a CU.
     if constexpr (K == 4) {
-        // Stack depth: more slots keep more results in LDS, fewer let more
-        // blocks share a CU.
         if (depth <= 2)
             launch_trav4<CM, 2>(entries, block_entry, blk0, nblocks, stream);
         else if (depth <= 4)
@@ -769,8 +955,13 @@ static void launch_traverse(const Entry *entries, const int32_t *block_entry, in
         else
             launch_trav4<CM, 8>(entries, block_entry, blk0, nblocks, stream);
     } else {
-        hipLaunchKernelGGL((k_traverse<K, CM>), dim3(nblocks), dim3(kTile), 0, stream, entries,
-                           block_entry, blk0);
+        constexpr int KP = kpad(K);
+        if (depth <= 2)
+            launch_trav20<KP, CM, 2>(entries, block_entry, blk0, nblocks, stream);
+        else if (depth <= 4)
+            launch_trav20<KP, CM, 4>(entries, block_entry, blk0, nblocks, stream);
+        else
+            launch_trav20<KP, CM, 8>(entries, block_entry, blk0, nblocks, stream);
     }
 }
 
