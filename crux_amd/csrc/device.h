@@ -3,8 +3,9 @@
 // HBM layout (all FP32 CLVs are site-major like the reference's cLMat,
 // lib/Crux/Tree/CxLik.h:10-36, but FP32 with an int32 power-of-two exponent
 // per site instead of an FP64 lnScale):
-//   CLV slot   : float  clv[S][C][KP]  then   int32 exp[S]   (K != 4; KP = K
-//                rounded up to a multiple of 4, states >= K held at 0)
+//   CLV slot   : float  clv[S/G][KP/4][64][4]  then  int32 exp[S]   (K != 4;
+//                KP = K rounded up to a multiple of 4, states >= K held at 0;
+//                G = 64/CM sites per tile, lane c*G + s%G as in the kernel)
 //                float  clv[C][S][4]   then   int32 exp[S]   (K == 4: component
 //                planes, so a wave's float4 access covers 1 KiB contiguously)
 //   tip codes  : uint8  codes[ntaxa][S]      index into the tree's code table

@@ -288,7 +288,13 @@ static int arena_layout(crux_tree *t, int pol, int C)
         return CRUX_OK;
     arena_reset(t, pol);
     a.C = C;
-    a.clv_bytes = align_up((size_t)t->S * C * kpad(t->K) * sizeof(float), 256);
+    if (t->K == 4) {
+        a.clv_bytes = align_up((size_t)t->S * C * 4 * sizeof(float), 256);
+    } else {
+        // Whole tiles of 64 (site, component) lanes of the component class.
+        const int cm = traverse_cmax(t->K, C), g = sites20(cm > 0 ? cm : 1);
+        a.clv_bytes = align_up((size_t)(t->S + g - 1) / g * 64 * kpad(t->K) * sizeof(float), 256);
+    }
     a.slot_bytes = a.clv_bytes + align_up(t->exp_count(C) * sizeof(int32_t), 256);
     a.chunk_slots = 0;
     return CRUX_OK;

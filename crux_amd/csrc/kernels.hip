@@ -581,7 +581,10 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 // and handed to the consumer in registers (the update just before it) or
 // through the wave's LDS stack, and the root update reduces to the
 // pattern-weighted FP64 site lnL (NaN -> -inf, CxLik.c:598-632) and a
-// compensated sum per wave.  Arithmetic follows the generic stripe
+// compensated sum per wave.  A slot holds the wave's registers as they are,
+// [S / G][KP / 4][64 lanes][4] (lanes of components >= C unused), so every
+// store and load of a CLV moves 1 KiB contiguously.  Arithmetic follows the
+// generic stripe
 // CxLikExecuteStripe (CxLik.c:635-846) with FP32 storage and power-of-two
 // scaling.
 //
@@ -602,8 +605,46 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 // of an update are loaded two updates ahead (one dword per lane, read out
 // with readlane).  A third child (the root update and chained
 // multifurcations) is loaded where it is used.
+// Raw buffer access (SGPR descriptor + per-lane offset + SGPR offset): the
+// per-load offset arithmetic stays on the scalar unit and no 64-bit address
+// is formed per lane.  Every access here is in bounds by construction.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ crux_f32x4 bload4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(crux_f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ int32_t bload1(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+
+__device__ __forceinline__ void bstore4(crux_f32x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff, 0);
+}
+
+__device__ __forceinline__ void bstore1(int32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, r, (int)voff, (int)soff, 0);
+}
+
+// Waves per SIMD the register allocation is held to: two where the registers
+// fit without spilling (up to 256 per lane), so that a chip's worth of waves
+// is resident at once.
+template <int KP, int CM>
+__host__ __device__ constexpr int trav20_waves()
+{
+    return KP == 20 && CM <= 4 ? 2 : 1;
+}
+
 template <int KP, int CM, int D>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, (trav20_waves<KP, CM>()))
 k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
     constexpr int G = sites20(CM);                  // sites per wave
@@ -629,7 +670,6 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     const int s = tile * G + t;
     const bool site_ok = s < S;
     const int sc = site_ok ? s : S - 1;     // in-bounds site for loads of idle lanes
-    const int cc = c < C ? c : C - 1;       // in-bounds component for idle lanes
     const CRUX_GAS int32_t *ctl = gas(E->ctl);
     const char *zero = reinterpret_cast<const char *>(E->zero);
     double sum = 0.0;
@@ -670,6 +710,8 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     // hx/he: the child's CLV and exponent when it comes from HBM.  Every load
     // is issued whatever the child is -- missing targets read the zero
     // block -- so no register is merged across branches.
+    const uint32_t lane16 = (uint32_t)lane * 16;
+    const uint8_t *tipb = reinterpret_cast<const uint8_t *>(&tipw[0][0]);
     auto fetch = [&](int32_t cw, bool ok, int j, crux_f32x4 (&pr)[PQ], crux_f32x4 (&hxr)[NI],
                      int32_t &her) {
         const int w0 = ok ? word(cw, 0) : 0;
@@ -679,22 +721,23 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         const bool hbm = have && kind != kChildTip && !((via >> j) & 1);
         const bool tip = kind == kChildTip;
         const char *mat = have && kind != kChildIdent ? ptr(cw, 8 + 6 * j) : zero;
-        const char *clv = hbm ? ptr(cw, 10 + 6 * j) : zero;
+        const char *clv = hbm ? ptr(cw, 10 + 6 * j) + (size_t)tile * NI * 1024 : zero;
         const char *exs = hbm ? ptr(cw, 12 + 6 * j) : zero;
         const int pos = op_pos(ok ? word(cw, 1) : 0, j) & (kTipChunk - 1);
-        const uint32_t code = (tipw[pos][t >> 2] >> (8 * (t & 3))) & 0xffu;
+        const uint32_t code = tipb[pos * G + t];
+        // float4 q of the lane: a tip row is NI consecutive float4 at the
+        // row of its (code, component); an image NQ float4 a wave apart.
+        const uint32_t voff = tip ? (code * CM + c) * (KP * 4) : lane16;
+        const auto rm = rsrc_of(mat), rh = rsrc_of(clv), re = rsrc_of(exs);
 #pragma unroll
-        for (int q = 0; q < PQ; ++q) {
-            const size_t off = tip ? ((size_t)(code * CM + c) * KP + 4 * (q < NI ? q : 0)) * 4
-                                   : (size_t)((q < NQ ? q : 0) * 64 + lane) * 16;
-            pr[q] = *(const CRUX_GAS crux_f32x4 *)gas(mat + off);
-        }
+        for (int q = 0; q < PQ; ++q)
+            pr[q] = bload4(rm, voff, tip ? (q < NI ? q : NI - 1) * 16 : (q < NQ ? q : NQ - 1) * 1024);
+        // The zero block reads as a slot would: lanes 16 B apart, float4 q
+        // a wave further.
 #pragma unroll
-        for (int q = 0; q < NI; ++q) {
-            const size_t off = hbm ? ((size_t)(sc * C + cc) * KP + 4 * q) * 4 : (size_t)q * 16;
-            hxr[q] = *(const CRUX_GAS crux_f32x4 *)gas(clv + off);
-        }
-        her = *(const CRUX_GAS int32_t *)gas(exs + (hbm ? (size_t)sc * 4 : 0));
+        for (int q = 0; q < NI; ++q)
+            hxr[q] = bload4(rh, lane16, q * 1024);
+        her = bload1(re, hbm ? (uint32_t)sc * 4 : 0u, 0);
     };
 
     // y = P_c x for every lane's own (site, component).
@@ -798,7 +841,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                         ex += hej;
                     }
                     if (kind == kChildIdent) {
-                        // Used as is; idle components read a real one's CLV.
+                        // Used as is; idle components hold no CLV.
 #pragma unroll
                         for (int I = 0; I < NI; ++I)
                             y[I] = c < C ? x[I] : crux_f32x4{0.f, 0.f, 0.f, 0.f};
@@ -843,23 +886,24 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                     stk[pp][I][lane] = yl[I];
                 stke[pp][lane] = el;
             }
-            if (site_ok && c < C) {
-                CRUX_GAS crux_f32x4 *dst = (CRUX_GAS crux_f32x4 *)gas(ptr(cw0, 4)) +
-                                           ((size_t)s * C + c) * NI;
+            // One float4 of every lane per store: 1 KiB contiguous.
+            if (c < C) {
+                const auto rd = rsrc_of(ptr(cw0, 4) + (size_t)tile * NI * 1024);
 #pragma unroll
                 for (int I = 0; I < NI; ++I)
-                    dst[I] = yl[I];
-                if (c == 0)
-                    ((CRUX_GAS int32_t *)gas(ptr(cw0, 6)))[s] = el;
+                    bstore4(yl[I], rd, lane16, I * 1024);
+                if (c == 0 && site_ok)
+                    bstore1(el, rsrc_of(ptr(cw0, 6)), (uint32_t)t * 4, (uint32_t)tile * G * 4);
             }
         } else {
+            // Weights of the lane's component (all loads issued at once;
+            // idle components read component 0's and add nothing).
+            const CRUX_GAS double *pw = gas(E->piw) + (size_t)(c < C ? c : 0) * K;
             double L = 0.0;
-            if (c < C) {
-                const CRUX_GAS double *pw = gas(E->piw) + (size_t)c * K;
 #pragma unroll
-                for (int i = 0; i < KP; ++i)
-                    if (i < K)
-                        L += pw[i] * (double)acc[i >> 2][i & 3];
+            for (int i = 0; i < KP; ++i) {
+                const double w = pw[i < K ? i : K - 1];
+                L += (i < K && c < C ? w : 0.0) * (double)acc[i >> 2][i & 3];
             }
             // Sum over the site's components; every lane ends with the same
             // value (each pairwise sum is commutative).
