@@ -634,6 +634,60 @@ __device__ __forceinline__ void bstore1(int32_t v, __amdgpu_buffer_rsrc_t r, uin
     __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, r, (int)voff, (int)soff, 0);
 }
 
+// The values of a lane and of its partner lane ^ OFF, as an unordered pair
+// (the pair is the same, in some order, in both lanes).  Lanes 32 or 16
+// apart swap halves of the wave on the vector ALU (v_permlane32_swap /
+// v_permlane16_swap); closer lanes go through ds_bpermute.
+template <int OFF>
+__device__ __forceinline__ void lane_pair(uint32_t v, uint32_t &a, uint32_t &b)
+{
+    if constexpr (OFF == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (OFF == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        a = r[0];
+        b = r[1];
+    } else {
+        a = v;
+        b = (uint32_t)__shfl_xor((int)v, OFF, 64);
+    }
+}
+
+// Reductions over the lanes G, 2G, ... apart (the components of a site).
+template <int G>
+__device__ __forceinline__ int32_t comp_max(int32_t v)
+{
+    static_for<6>([&](auto K) {
+        constexpr int off = 1 << decltype(K)::value;
+        if constexpr (off >= G) {
+            uint32_t a, b;
+            lane_pair<off>((uint32_t)v, a, b);
+            v = max((int32_t)a, (int32_t)b);
+        }
+    });
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ double comp_sum(double v)
+{
+    static_for<6>([&](auto K) {
+        constexpr int off = 1 << decltype(K)::value;
+        if constexpr (off >= G) {
+            const uint64_t u = (uint64_t)__double_as_longlong(v);
+            uint32_t al, bl, ah, bh;
+            lane_pair<off>((uint32_t)u, al, bl);
+            lane_pair<off>((uint32_t)(u >> 32), ah, bh);
+            // Commutative, so both lanes of a pair get the same sum.
+            v = __longlong_as_double((long long)(((uint64_t)ah << 32) | al)) +
+                __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
+        }
+    });
+    return v;
+}
+
 // Waves per SIMD the register allocation is held to: two where the registers
 // fit without spilling (up to 256 per lane), so that a chip's worth of waves
 // is resident at once.
@@ -867,9 +921,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             for (int I = 0; I < NI; ++I)
                 mb = max(max(mb, max(__float_as_int(acc[I].x), __float_as_int(acc[I].y))),
                          max(__float_as_int(acc[I].z), __float_as_int(acc[I].w)));
-#pragma unroll
-            for (int off = G; off < 64; off <<= 1)
-                mb = max(mb, __shfl_xor(mb, off, 64));
+            mb = comp_max<G>(mb);
             const float m = __int_as_float(mb);
             int e2 = 0;
             if (m > 0.0f)
@@ -907,9 +959,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             }
             // Sum over the site's components; every lane ends with the same
             // value (each pairwise sum is commutative).
-#pragma unroll
-            for (int off = G; off < 64; off <<= 1)
-                L += __shfl_xor(L, off, 64);
+            L = comp_sum<G>(L);
             double v = (log(L) + (double)ex * 0.69314718055994530942) *
                        (double)gas(E->weights)[sc];
             if (__builtin_isnan(v))

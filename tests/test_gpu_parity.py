@@ -94,6 +94,49 @@ def test_random_trees_vs_oracle(ntaxa, nsites, seed, kind):
         assert close(a, b), (i, a, b)
 
 
+@pytest.mark.parametrize("ncat,kind", [(1, "aa"), (2, "aa"), (3, "aa"), (6, "aa"), (2, "protein")])
+def test_state_and_component_classes(ncat, kind):
+    # Every component class of the 20/22-state kernel (1, 2, 4 with an idle
+    # component, 8), with every node as root: the roots after the first
+    # re-evaluate from cached CLVs read back from HBM.
+    say("classes", ncat, kind)
+    rng = np.random.default_rng(60 + ncat)
+    if kind == "aa":
+        prob = synth.make_problem(18, 600, 60 + ncat, kind="aa", ncat=ncat)
+    else:
+        rooted, tree = synth.yule_tree(14, rng, 0.15)
+        m = synth.Model(rng.lognormal(0, 1, 231), rng.dirichlet(np.ones(22)), 0.6, ncat, Protein)
+        aln = synth.simulate(rooted, m, 500, rng)
+        aln.compact()
+        prob = synth.Problem(tree, rooted, aln, m, 500, "")
+    dev, ora = _pair(prob, ncat=ncat)
+    a, b = dev.lnL(), ora.lnL()
+    assert close(a, b), (a, b)
+    sd, so = np.array(dev.siteLnLs()), np.array(ora.siteLnLs())
+    assert np.allclose(sd, so, rtol=1e-6, atol=1e-5)
+    for node in list(dev.tree.nodes)[::2]:
+        assert close(dev.lnL(node), b), node
+
+
+def test_aa_partial_reevaluation():
+    # Branch-length changes on a 20-state tree: each evaluation recomputes the
+    # path to the changed edge, the siblings' CLVs come from HBM.
+    prob = synth.make_problem(40, 900, 22, kind="aa")
+    dev, ora = _pair(prob, ncat=4)
+    dev.lnL()
+    rng = np.random.default_rng(6)
+    edges_d = list(dev.tree.edges)
+    edges_o = list(ora.tree.edges)
+    for step in range(10):
+        k = int(rng.integers(len(edges_d)))
+        f = math.exp(2 * math.log(1.6) * (rng.random() - 0.5))
+        edges_d[k].length = edges_d[k].length * f
+        edges_o[k].length = edges_o[k].length * f
+        a = dev.lnL(edges_d[k].ring.node)
+        b = ora.lnL(edges_o[k].ring.node)
+        assert close(a, b), (step, a, b)
+
+
 def test_compute_dtypes_full_size_c1():
     # BASELINE config 1 shape: 16 taxa, 1000 sites, GTR+G4 alpha=0.5.
     prob = synth.make_problem(16, 1000, 1)
