@@ -3,9 +3,13 @@
 Workload: 256-taxon Yule tree, 20,000 simulated sites compressed to
 patterns, random reversible 20x20 exchangeabilities ~LogNormal(0,1), freqs
 ~Dirichlet(1 x 20), +G4 (alpha 0.5).  Reports full-traversal lnL evals/s,
-the traversal kernel time (HIP events), its matrix-core (or vector) FP32
-throughput against the MI355X peak, |dlnL| against the FP64 oracle and the
-oracle's CPU throughput on a bounded sample.
+the traversal kernel time (HIP events), its roofline -- algorithmic HBM
+bytes (CLV and exponent writes, tip codes, weights in, site lnLs out) per
+launch against 8 TB/s, and the matrix-core FP32 work (20x20 P times CLV per
+internal child edge, component and pattern) against 157.3 TFLOP/s; at about
+10 flop/B the kernel sits below the ~20 flop/B ridge, so HBM bounds it --
+|dlnL| against the FP64 oracle and the oracle's CPU throughput on a bounded
+sample.
 
     python -m crux_amd.bench_aa [--steps 20] [--no-cpu]
 """
@@ -25,6 +29,7 @@ from . import synth
 from .lik import Lik
 
 FP32_MATRIX_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
 
 
 def main(argv=None):
@@ -62,14 +67,16 @@ def main(argv=None):
         tr.append(ctx.last_timing()[1])
     ctx.set_timing(False)
     trav = float(np.mean(tr))
-    _, _, nodes = ctx.last_traffic()
-    # Useful FP32 flops: per node update, per component and child edge, a
-    # 20x20 matrix times a 20-vector per pattern (2 flops per multiply-add).
-    nedge = 2 * prob.alignment.ntaxa - 3
-    flops = 2.0 * 20 * 20 * S * 4 * nedge
+    nbytes, _, nodes = ctx.last_traffic()
+    # Matrix-core FP32 work: per internal child edge and component, a 20x20
+    # matrix times a 20-vector per pattern (2 flops per multiply-add); tip
+    # children are table rows (no product).
+    ninternal = prob.alignment.ntaxa - 3
+    flops = 2.0 * 20 * 20 * S * 4 * ninternal
     tflops = flops / (trav * 1e-3) / 1e12
-    print("full: %.1f evals/s; traverse %.3f ms; %.1f TFLOP/s" % (eps, trav, tflops),
-          file=sys.stderr, flush=True)
+    gbs = nbytes / (trav * 1e-3) / 1e9
+    print("full: %.1f evals/s; traverse %.3f ms; %.1f GB/s (%.1f%% of HBM); %.1f TFLOP/s" % (
+        eps, trav, gbs, 100 * gbs / HBM_PEAK_GBS, tflops), file=sys.stderr, flush=True)
     cpu = None
     dl = None
     if not args.no_cpu:
@@ -92,9 +99,14 @@ def main(argv=None):
     print(json.dumps({"workload": "amino acid: 256 taxa, 20,000 sites -> %d patterns, "
                                   "LogNormal exchangeabilities, Dirichlet(1) freqs, +G4" % S,
                       "evals_per_s": round(eps, 2), "traverse_ms": round(trav, 4),
-                      "node_updates": nodes, "useful_tflops": round(tflops, 2),
-                      "peak_tflops_fp32_matrix": FP32_MATRIX_PEAK_TFLOPS,
-                      "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4), "lnL": lnl,
+                      "node_updates": nodes,
+                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                   "bytes": nbytes},
+                      "matrix_core": {"tflops": round(tflops, 2),
+                                      "peak_tflops_fp32": FP32_MATRIX_PEAK_TFLOPS,
+                                      "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4)},
+                      "lnL": lnl,
                       "dlnL": dl, "cpu_baseline": cpu}), flush=True)
     return 0
 
