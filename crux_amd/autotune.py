@@ -1,12 +1,13 @@
 """Measured choice of the traversal's launch-shape knobs.
 
-The 4-state traversal keeps the results a parent update has not consumed yet
+The traversal kernels keep the results a parent update has not consumed yet
 in a per-lane LDS stack of 2, 4 or 8 slots.  More slots keep more results
 out of HBM; fewer let more blocks share a CU.  Which wins depends on the tree
-shape, the component count and the pattern count, so the tuner times full
-traversals of the caller's own likelihood at each depth (with the device's
-own event timer around the traversal launch) and leaves the fastest set on
-the likelihood's context.
+shape, the state and component counts and the pattern count, so the tuner
+times full traversals of the caller's own likelihood at each depth (with the
+device's own event timer around the traversal launch) and leaves the fastest
+set on the likelihood's context (the setting is per context: it applies to
+every likelihood evaluated there).
 
     from crux_amd.autotune import tune
     best, times = tune(lik)          # times: {depth: traversal ms}

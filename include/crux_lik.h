@@ -182,9 +182,10 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
 
 /* Launch-shape tuning of the context (set by the autotuner,
  * crux_amd/autotune.py):
- *   CRUX_TUNE_STACK_DEPTH: LDS stack slots per lane of the 4-state kernel
- *   (2, 4 or 8; 0 = the largest that fits).  Fewer slots let more blocks
- *   share a CU; more slots keep more pending results out of HBM. */
+ *   CRUX_TUNE_STACK_DEPTH: LDS stack slots per lane of the traversal
+ *   kernels (2, 4 or 8, capped by what fits; 0 = default: 8 for 4 states
+ *   with up to 2 components, else 4).  Fewer slots let more blocks share a
+ *   CU; more slots keep more pending results out of HBM. */
 enum { CRUX_TUNE_STACK_DEPTH = 1 };
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
