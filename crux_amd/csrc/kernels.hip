@@ -187,6 +187,89 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+// Raw buffer access (SGPR descriptor + per-lane offset + SGPR offset): the
+// per-load offset arithmetic stays on the scalar unit and no 64-bit address
+// is formed per lane.  Every access here is in bounds by construction.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ crux_f32x4 bload4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(crux_f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ int32_t bload1(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+
+__device__ __forceinline__ void bstore4(crux_f32x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff, 0);
+}
+
+__device__ __forceinline__ void bstore1(int32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, r, (int)voff, (int)soff, 0);
+}
+
+// The values of a lane and of its partner lane ^ OFF, as an unordered pair
+// (the pair is the same, in some order, in both lanes).  Lanes 32 or 16
+// apart swap halves of the wave on the vector ALU (v_permlane32_swap /
+// v_permlane16_swap); closer lanes go through ds_bpermute.
+template <int OFF>
+__device__ __forceinline__ void lane_pair(uint32_t v, uint32_t &a, uint32_t &b)
+{
+    if constexpr (OFF == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (OFF == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        a = r[0];
+        b = r[1];
+    } else {
+        a = v;
+        b = (uint32_t)__shfl_xor((int)v, OFF, 64);
+    }
+}
+
+// Reductions over the lanes G, 2G, ... apart (the components of a site).
+template <int G>
+__device__ __forceinline__ int32_t comp_max(int32_t v)
+{
+    static_for<6>([&](auto K) {
+        constexpr int off = 1 << decltype(K)::value;
+        if constexpr (off >= G) {
+            uint32_t a, b;
+            lane_pair<off>((uint32_t)v, a, b);
+            v = max((int32_t)a, (int32_t)b);
+        }
+    });
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ double comp_sum(double v)
+{
+    static_for<6>([&](auto K) {
+        constexpr int off = 1 << decltype(K)::value;
+        if constexpr (off >= G) {
+            const uint64_t u = (uint64_t)__double_as_longlong(v);
+            uint32_t al, bl, ah, bh;
+            lane_pair<off>((uint32_t)u, al, bl);
+            lane_pair<off>((uint32_t)(u >> 32), ah, bh);
+            // Commutative, so both lanes of a pair get the same sum.
+            v = __longlong_as_double((long long)(((uint64_t)ah << 32) | al)) +
+                __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
+        }
+    });
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // 4-state helpers.
 
@@ -285,11 +368,11 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
     auto fetch = [&](int k) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const int i = q * kTile4 + lane, g = k * kChunkRows + i;
-            if (i < kChunkRows && g < total_r) {
-                const crux_f32x4 v = *(const CRUX_GAS crux_f32x4 *)(pd + g);
-                pf[q] = make_float4(v.x, v.y, v.z, v.w);
-            }
+            // Rows past the chunk or the plan load a real row (never read).
+            const int i = min(q * kTile4 + lane, kChunkRows - 1);
+            const int g = min(k * kChunkRows + i, total_r - 1);
+            const crux_f32x4 v = *(const CRUX_GAS crux_f32x4 *)(pd + g);
+            pf[q] = make_float4(v.x, v.y, v.z, v.w);
         }
         if (lane < PC * kCtlWords)
             cf = (k * PC + lane / kCtlWords < nops) ? cg[k * PC * kCtlWords + lane] : 0;
@@ -487,12 +570,14 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                 // stays inside this branch (see stage_tips).
                 const CRUX_CAS OpChild &ch = op->ch[j];
                 float4 x[CM];
+                const auto rc = rsrc_of(ch.clv);
 #pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    x[c] = c < C ? ld4(gas(reinterpret_cast<const float4 *>(ch.clv)) +
-                                       (size_t)c * S + sc)
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
-                ex += gas(ch.exps)[sc];
+                for (int c = 0; c < CM; ++c) {
+                    // Components past C read component C - 1 (in the slot).
+                    const crux_f32x4 v = bload4(rc, (uint32_t)sc * 16, (uint32_t)(min(c, C - 1) * S) * 16);
+                    x[c] = c < C ? make_float4(v.x, v.y, v.z, v.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+                ex += bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
                 __builtin_amdgcn_s_waitcnt(0);
                 take(j, [&](int c) { return x[c]; }, mat);
             }
@@ -526,11 +611,13 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                 stke[pp][lane] = el;
             }
             if (act) {
+                const auto rp = rsrc_of(reinterpret_cast<const void *>(par));
 #pragma unroll
                 for (int c = 0; c < CM; ++c)
                     if (c < C)
-                        st4(gas(reinterpret_cast<float4 *>(par)) + (size_t)c * S + s, yl[c]);
-                gas(reinterpret_cast<int32_t *>(pex))[s] = el;
+                        bstore4(crux_f32x4{yl[c].x, yl[c].y, yl[c].z, yl[c].w}, rp,
+                                (uint32_t)s * 16, (uint32_t)(c * S) * 16);
+                bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex)), (uint32_t)s * 4, 0);
             }
         } else {
             double L = 0.0;
@@ -605,89 +692,6 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 // of an update are loaded two updates ahead (one dword per lane, read out
 // with readlane).  A third child (the root update and chained
 // multifurcations) is loaded where it is used.
-// Raw buffer access (SGPR descriptor + per-lane offset + SGPR offset): the
-// per-load offset arithmetic stays on the scalar unit and no 64-bit address
-// is formed per lane.  Every access here is in bounds by construction.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *p)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
-}
-
-__device__ __forceinline__ crux_f32x4 bload4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    return __builtin_bit_cast(crux_f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
-}
-
-__device__ __forceinline__ int32_t bload1(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
-}
-
-__device__ __forceinline__ void bstore4(crux_f32x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff, 0);
-}
-
-__device__ __forceinline__ void bstore1(int32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, r, (int)voff, (int)soff, 0);
-}
-
-// The values of a lane and of its partner lane ^ OFF, as an unordered pair
-// (the pair is the same, in some order, in both lanes).  Lanes 32 or 16
-// apart swap halves of the wave on the vector ALU (v_permlane32_swap /
-// v_permlane16_swap); closer lanes go through ds_bpermute.
-template <int OFF>
-__device__ __forceinline__ void lane_pair(uint32_t v, uint32_t &a, uint32_t &b)
-{
-    if constexpr (OFF == 32) {
-        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        a = r[0];
-        b = r[1];
-    } else if constexpr (OFF == 16) {
-        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        a = r[0];
-        b = r[1];
-    } else {
-        a = v;
-        b = (uint32_t)__shfl_xor((int)v, OFF, 64);
-    }
-}
-
-// Reductions over the lanes G, 2G, ... apart (the components of a site).
-template <int G>
-__device__ __forceinline__ int32_t comp_max(int32_t v)
-{
-    static_for<6>([&](auto K) {
-        constexpr int off = 1 << decltype(K)::value;
-        if constexpr (off >= G) {
-            uint32_t a, b;
-            lane_pair<off>((uint32_t)v, a, b);
-            v = max((int32_t)a, (int32_t)b);
-        }
-    });
-    return v;
-}
-
-template <int G>
-__device__ __forceinline__ double comp_sum(double v)
-{
-    static_for<6>([&](auto K) {
-        constexpr int off = 1 << decltype(K)::value;
-        if constexpr (off >= G) {
-            const uint64_t u = (uint64_t)__double_as_longlong(v);
-            uint32_t al, bl, ah, bh;
-            lane_pair<off>((uint32_t)u, al, bl);
-            lane_pair<off>((uint32_t)(u >> 32), ah, bh);
-            // Commutative, so both lanes of a pair get the same sum.
-            v = __longlong_as_double((long long)(((uint64_t)ah << 32) | al)) +
-                __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
-        }
-    });
-    return v;
-}
-
 // Waves per SIMD the register allocation is held to: two where the registers
 // fit without spilling (up to 256 per lane), so that a chip's worth of waves
 // is resident at once.
