@@ -90,8 +90,8 @@ struct crux_ctx {
         size_t pmat_cap = 0;           // floats
         double *d_partials = nullptr;  // [sum ntiles][2]
         size_t partials_cap = 0;       // doubles
-        double *d_lnl = nullptr;
-        double *h_lnl = nullptr;       // pinned
+        double *h_lnl = nullptr;       // pinned, coherent: the reduction writes it
+        double *d_lnl = nullptr;       // the device's address of h_lnl
         size_t lnl_cap = 0;
         hipEvent_t done = nullptr;
         bool pending = false;
@@ -186,9 +186,23 @@ struct PlanChild {
     double len;
 };
 
+// The children of one fused update (at most kOpMaxChild), held inline.
+struct ChildList {
+    PlanChild v[kOpMaxChild];
+    int n = 0;
+    void push_back(const PlanChild &c) { v[n++] = c; }
+    size_t size() const { return (size_t)n; }
+    PlanChild &operator[](size_t i) { return v[i]; }
+    const PlanChild &operator[](size_t i) const { return v[i]; }
+    PlanChild *begin() { return v; }
+    PlanChild *end() { return v + n; }
+    const PlanChild *begin() const { return v; }
+    const PlanChild *end() const { return v + n; }
+};
+
 struct PlanOp {
     int32_t parent_slot;   // -1: root reduce
-    std::vector<PlanChild> ch;
+    ChildList ch;
     int32_t from_stack = 0;            // bit j: child j comes from the LDS stack
     int32_t fwd = 0;                   // bit j: child j is the previous update's result
     int8_t pos[kOpMaxChild + 1] = {-1, -1, -1, -1};
@@ -312,6 +326,7 @@ struct Planner {
     bool inval;
     std::vector<PlanOp> &ops;
     int rc = CRUX_OK;
+    std::vector<PlanChild> chbuf;   // children of the node being emitted
 
     bool &valid_of(int32_t key)
     {
@@ -423,7 +438,8 @@ struct Planner {
             me.elen = elen;
         }
         if (deg != 1 && (inval || !me.valid)) {
-            std::vector<PlanChild> ch;
+            std::vector<PlanChild> &ch = chbuf;
+            ch.clear();
             for (int32_t r = t->ring_next[ring]; r != ring; r = t->ring_next[r]) {
                 double len = t->edge_len[r >> 1];
                 if (len < 0.0) {
@@ -459,7 +475,10 @@ struct Planner {
                     rc = set_error(CRUX_ENEGLEN, "Negative branch length");
                     return;
                 }
-                std::vector<PlanChild> ch{child_of(ring ^ 1, len), child_of(ring, 0.0)};
+                std::vector<PlanChild> &ch = chbuf;
+                ch.clear();
+                ch.push_back(child_of(ring ^ 1, len));
+                ch.push_back(child_of(ring, 0.0));
                 emit(-1, ch);
                 L->root_valid = true;
             }
@@ -474,7 +493,8 @@ struct Planner {
                 r = t->ring_next[r];
             } while (r != ring);
             if (inval || !L->root_valid) {
-                std::vector<PlanChild> ch;
+                std::vector<PlanChild> &ch = chbuf;
+                ch.clear();
                 r = ring;
                 do {
                     double len = t->edge_len[r >> 1];
@@ -542,54 +562,69 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
     const int n = (int)ops.size();
     if (n == 0)
         return;
+    // Scratch reused across calls (the planner runs under the context lock;
+    // thread_local keeps contexts on different threads apart).
+    thread_local std::vector<int> writer;                  // slot -> latest op writing it
+    thread_local std::vector<std::array<int, kOpMaxChild>> producer;
+    thread_local std::vector<int> consumer, need, order, push_slot, kidn;
+    thread_local std::vector<std::array<int, kOpMaxChild>> kids;
+    thread_local std::vector<std::pair<int, int>> st;
+    thread_local std::vector<char> forwarded;
+    thread_local std::vector<PlanOp> out;
     // producer[i][j]: op that produced child j of op i in this plan, or -1.
-    std::vector<std::array<int, kOpMaxChild>> producer(n);
-    std::vector<int> consumer(n, -1);
-    {
-        std::unordered_map<int32_t, int> writer;   // slot -> latest op writing it
-        for (int i = 0; i < n; i++) {
-            for (int j = 0; j < kOpMaxChild; j++)
-                producer[i][j] = -1;
-            for (size_t j = 0; j < ops[i].ch.size(); j++) {
-                const PlanChild &c = ops[i].ch[j];
-                if (c.kind == kChildTip)
-                    continue;
-                auto w = writer.find(c.slot);
-                if (w != writer.end() && consumer[w->second] < 0) {
-                    producer[i][j] = w->second;
-                    consumer[w->second] = i;
-                }
-            }
-            if (ops[i].parent_slot >= 0)
-                writer[ops[i].parent_slot] = i;
-        }
+    int32_t maxslot = -1;
+    for (int i = 0; i < n; i++) {
+        maxslot = std::max(maxslot, ops[i].parent_slot);
+        for (const PlanChild &c : ops[i].ch)
+            maxslot = std::max(maxslot, c.slot);
     }
-    // Stack need per op (children are earlier in the original order).
-    std::vector<int> need(n, 1);
-    std::vector<std::vector<int>> kids(n);
+    writer.assign((size_t)maxslot + 1, -1);
+    producer.resize(n);
+    consumer.assign(n, -1);
     for (int i = 0; i < n; i++) {
         for (int j = 0; j < kOpMaxChild; j++)
+            producer[i][j] = -1;
+        for (size_t j = 0; j < ops[i].ch.size(); j++) {
+            const PlanChild &c = ops[i].ch[j];
+            if (c.kind == kChildTip || c.slot < 0)
+                continue;
+            const int w = writer[c.slot];
+            if (w >= 0 && consumer[w] < 0) {
+                producer[i][j] = w;
+                consumer[w] = i;
+            }
+        }
+        if (ops[i].parent_slot >= 0)
+            writer[ops[i].parent_slot] = i;
+    }
+    // Stack need per op (children are earlier in the original order); the
+    // producing children sorted deepest first.
+    need.assign(n, 1);
+    kids.resize(n);
+    kidn.assign(n, 0);
+    for (int i = 0; i < n; i++) {
+        int &k = kidn[i];
+        for (int j = 0; j < kOpMaxChild; j++)
             if (producer[i][j] >= 0)
-                kids[i].push_back(producer[i][j]);
-        std::stable_sort(kids[i].begin(), kids[i].end(),
+                kids[i][k++] = producer[i][j];
+        std::stable_sort(kids[i].begin(), kids[i].begin() + k,
                          [&](int a, int b) { return need[a] > need[b]; });
         int d = 1;
-        for (size_t k = 0; k < kids[i].size(); k++)
-            d = std::max(d, (int)k + need[kids[i][k]]);
+        for (int q = 0; q < k; q++)
+            d = std::max(d, q + need[kids[i][q]]);
         need[i] = d;
     }
     // Emit post-order from each root (ops without a consumer), deepest first.
-    std::vector<int> order;
-    order.reserve(n);
-    std::vector<std::pair<int, size_t>> st;
+    order.clear();
     for (int r = 0; r < n; r++) {
         if (consumer[r] >= 0)
             continue;
+        st.clear();
         st.push_back({r, 0});
         while (!st.empty()) {
             auto &top = st.back();
-            if (top.second < kids[top.first].size()) {
-                int k = kids[top.first][top.second++];
+            if (top.second < kidn[top.first]) {
+                const int k = kids[top.first][top.second++];
                 st.push_back({k, 0});
             } else {
                 order.push_back(top.first);
@@ -601,14 +636,13 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
     // very next update stays in the kernel's registers (forwarded) and takes
     // no slot; in post-order that is the last child of every update with an
     // internal child, so the stack only holds the earlier siblings.
-    std::vector<int> push_slot(n, -1);
-    std::vector<char> forwarded(n, 0);
-    std::vector<PlanOp> out;
-    out.reserve(n);
+    push_slot.assign(n, -1);
+    forwarded.assign(n, 0);
+    out.clear();
     int sp = 0;
     for (size_t k = 0; k < order.size(); k++) {
         const int i = order[k];
-        PlanOp op = std::move(ops[i]);
+        PlanOp op = ops[i];
         op.from_stack = 0;
         op.fwd = 0;
         for (int j = 0; j < kOpMaxChild; j++)
@@ -633,7 +667,7 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
             op.pos[kOpMaxChild] = (int8_t)sp;
             sp++;
         }
-        out.push_back(std::move(op));
+        out.push_back(op);
     }
     ops.swap(out);
 }
@@ -861,14 +895,14 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         size_t ncap = std::max(liks.size(), S.lnl_cap * 2);
         if (S.h_lnl)
             (void)hipHostFree(S.h_lnl);
-        dev_free(ctx, S.d_lnl, S.lnl_cap * sizeof(double));
         S.h_lnl = nullptr;
         S.d_lnl = nullptr;
         S.lnl_cap = 0;
-        HIP_TRY(hipHostMalloc((void **)&S.h_lnl, ncap * sizeof(double), hipHostMallocDefault));
-        rc = dev_alloc(ctx, (void **)&S.d_lnl, ncap * sizeof(double));
-        if (rc)
-            return rc;
+        // Results land in host memory straight from the reduction kernel
+        // (no copy back to enqueue).
+        HIP_TRY(hipHostMalloc((void **)&S.h_lnl, ncap * sizeof(double),
+                              hipHostMallocCoherent | hipHostMallocMapped));
+        HIP_TRY(hipHostGetDevicePointer((void **)&S.d_lnl, S.h_lnl, 0));
         S.lnl_cap = ncap;
     }
 
@@ -1095,8 +1129,6 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     HIP_TRY(hipGetLastError());
     if (ctx->timing)
         HIP_TRY(hipEventRecord(ctx->ev[3], st));
-    HIP_TRY(hipMemcpyAsync(S.h_lnl, S.d_lnl, liks.size() * sizeof(double),
-                           hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(S.done, st));
     ctx->last_bytes = traffic;
     ctx->last_formula = formula;
@@ -1222,7 +1254,6 @@ int crux_ctx_destroy(crux_ctx *ctx)
         dev_free(ctx, S.d_plan, S.plan_cap);
         dev_free(ctx, S.d_pmat, S.pmat_cap * sizeof(float));
         dev_free(ctx, S.d_partials, S.partials_cap * sizeof(double));
-        dev_free(ctx, S.d_lnl, S.lnl_cap * sizeof(double));
         if (S.done)
             (void)hipEventDestroy(S.done);
     }
