@@ -998,9 +998,23 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 __global__ void __launch_bounds__(64) k_finalize(const Entry *__restrict__ entries)
 {
     const Entry *E = &entries[blockIdx.x];
+    const int nt = E->ntiles, lane = (int)threadIdx.x;
+    const CRUX_GAS double *pp = gas(E->partials);
     double s = 0.0, c = 0.0;
-    for (int t = (int)threadIdx.x; t < E->ntiles; t += 64)
-        nsum_merge(s, c, E->partials[2 * t], E->partials[2 * t + 1]);
+    // Lane-strided over the tiles, eight loads in flight per lane (a zero
+    // pair past the end adds nothing).
+    for (int t0 = 0; t0 < nt; t0 += 8 * 64) {
+        double vs[8], vc[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = t0 + k * 64 + lane;
+            vs[k] = t < nt ? pp[2 * t] : 0.0;
+            vc[k] = t < nt ? pp[2 * t + 1] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            nsum_merge(s, c, vs[k], vc[k]);
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double s2 = __shfl_xor(s, off, 64), c2 = __shfl_xor(c, off, 64);
