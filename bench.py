@@ -144,7 +144,7 @@ def main():
     log("partial: %.1f evals/s" % part_eps)
 
     # --- FP64 oracle: parity and CPU baseline ---------------------------------
-    cpu = None
+    cpu = cpu1 = None
     dl_abs = dl_rel = None
     if not args.no_cpu:
         from oracle.oracle import OracleLik
@@ -171,6 +171,50 @@ def main():
                "sample": "%d full-traversal evaluations of the same 128-taxon, %d-pattern "
                          "GTR+G4 workload by the FP64 C restatement of the reference "
                          "engine (oracle/cxlik_oracle.c) with %d stripe threads" % (n, S, ncpu)}
+        # The same restatement on one thread (a few evaluations).
+        ora1 = OracleLik(prob2.fresh_tree(), prob2.alignment, ncat=4, nthreads=1)
+        synth.configure(ora1, prob2.model)
+        t = time.perf_counter()
+        ora1.lnL()
+        n1 = 1
+        while time.perf_counter() - t < args.cpu_seconds / 4 and n1 < 5:
+            ora1.lnL()
+            n1 += 1
+        cpu1 = {"value": round(n1 / (time.perf_counter() - t), 3), "unit": "lnL evals/s",
+                "cores": 1, "kind": "port", "sample": "%d evaluations, one thread" % n1}
+        log("oracle, one thread: %.2f evals/s" % cpu1["value"])
+
+    # Write bandwidth the HIP runtime's fill kernel reaches on this GPU
+    # (hipMemsetD32Async over 512 MiB, timed with HIP events on the null
+    # stream): the practical ceiling of an almost write-only kernel, next to
+    # the published peak.
+    write_peak = None
+    try:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = 512 << 20
+        buf, e0, e1 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        ok = hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) == 0
+        ok = ok and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+        if ok:
+            fill = lambda: hip.hipMemsetD32Async(buf, ctypes.c_int(0x3F800000),
+                                                 ctypes.c_size_t(nbytes // 4), None)
+            for _ in range(3):
+                fill()
+            hip.hipEventRecord(e0, None)
+            for _ in range(10):
+                fill()
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = ctypes.c_float()
+            hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+            write_peak = nbytes * 10 / (ms.value * 1e-3) / 1e9
+            hip.hipEventDestroy(e0)
+            hip.hipEventDestroy(e1)
+            hip.hipFree(buf)
+            log("fill kernel: %.1f GB/s" % write_peak)
+    except OSError as exc:
+        log("fill kernel not measured: %s" % exc)
 
     # HBM bytes per traversal launch as rocprofv3's FETCH_SIZE + WRITE_SIZE
     # counters read them for this workload (scaled by the copy-kernel
@@ -210,8 +254,11 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_hbm, "algorithmic_bytes": traffic, "node_updates": nodes,
                      "model_bytes": formula_bytes,
-                     "model_GBps": round(formula_bytes / (trav_ms * 1e-3) / 1e9, 1)},
+                     "model_GBps": round(formula_bytes / (trav_ms * 1e-3) / 1e9, 1),
+                     "measured_write_GBps": round(write_peak, 1) if write_peak else None,
+                     "frac_of_measured_write": round(achieved / write_peak, 4) if write_peak else None},
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
     }
     print(json.dumps(line), flush=True)
     return 0
