@@ -94,7 +94,8 @@ def test_random_trees_vs_oracle(ntaxa, nsites, seed, kind):
         assert close(a, b), (i, a, b)
 
 
-@pytest.mark.parametrize("ncat,kind", [(1, "aa"), (2, "aa"), (3, "aa"), (6, "aa"), (2, "protein")])
+@pytest.mark.parametrize("ncat,kind", [(1, "aa"), (2, "aa"), (3, "aa"), (6, "aa"), (1, "protein"),
+                                       (2, "protein"), (6, "protein")])
 def test_state_and_component_classes(ncat, kind):
     # Every component class of the 20/22-state kernel (1, 2, 4 with an idle
     # component, 8), with every node as root: the roots after the first
@@ -116,6 +117,18 @@ def test_state_and_component_classes(ncat, kind):
     assert np.allclose(sd, so, rtol=1e-6, atol=1e-5)
     for node in list(dev.tree.nodes)[::2]:
         assert close(dev.lnL(node), b), node
+
+
+def test_aa_many_tips():
+    # More tips than one staged chunk of tip codes (64): later chunks are
+    # staged where the plan reaches them.
+    say("aa many tips")
+    prob = synth.make_problem(150, 400, 23, kind="aa")
+    dev, ora = _pair(prob, ncat=4)
+    a, b = dev.lnL(), ora.lnL()
+    assert close(a, b), (a, b)
+    sd, so = np.array(dev.siteLnLs()), np.array(ora.siteLnLs())
+    assert np.allclose(sd, so, rtol=1e-6, atol=1e-5)
 
 
 def test_aa_partial_reevaluation():
