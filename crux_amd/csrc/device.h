@@ -8,7 +8,6 @@
 //                G = 64/CM sites per tile, lane c*G + s%G as in the kernel)
 //                float  clv[C][S][4]   then   int32 exp[S]   (K == 4: component
 //                planes, so a wave's float4 access covers 1 KiB contiguously)
-//   tip codes  : uint8  codes[ntaxa][S]      index into the tree's code table
 //   tip rows   : uint8  rows[ntaxa][S64]     K == 4: 4-bit state set; K != 4:
 //                the code index; rows padded to S64 = S rounded up to 64
 //   P block    : float  P[CM][4][4]          K == 4, row = parent state
@@ -36,7 +35,7 @@ enum ChildKind : int32_t {
 struct OpChild {
     const float *clv;      // internal/ident: child CLV slot
     const int32_t *exps;   // internal/ident: child exponents
-    const uint8_t *codes;  // tip: code row of the taxon
+    const uint8_t *codes;  // tip: padded tip row of the taxon
     const float *mat;      // internal: P block; tip: tip table
 };
 

@@ -216,7 +216,6 @@ struct PlanOp {
 struct crux_tree {
     crux_ctx *ctx = nullptr;
     int K = 0, S = 0, ntaxa = 0, ncodes = 0;
-    uint8_t *d_codes = nullptr;
     double *d_codevecs = nullptr;
     uint8_t *d_masks = nullptr;     // padded tip rows: 4-bit state mask (K == 4) or code
     uint32_t *d_weights = nullptr;
@@ -998,9 +997,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                 OpChild &oc = op.ch[j];
                 op.kinds |= pc.kind << (2 * j);
                 if (pc.kind == kChildTip) {
-                    // 4-state kernel: state masks; otherwise code indices.
-                    oc.codes = K == 4 ? t->d_masks + (size_t)pc.taxon * t->mask_stride()
-                                      : t->d_codes + (size_t)pc.taxon * t->S;
+                    // Padded tip row: state masks (4 states) or code indices.
+                    oc.codes = t->d_masks + (size_t)pc.taxon * t->mask_stride();
                     traffic += (uint64_t)t->S;
                     formula += (uint64_t)t->S;
                 } else {
@@ -1353,19 +1351,15 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
     t->S = nsites;
     t->ntaxa = ntaxa;
     t->ncodes = ncodes;
-    int rc = dev_alloc(ctx, (void **)&t->d_codes, (size_t)ntaxa * nsites);
-    if (!rc)
-        rc = dev_alloc(ctx, (void **)&t->d_codevecs, sizeof(double) * ncodes * dim);
+    int rc = dev_alloc(ctx, (void **)&t->d_codevecs, sizeof(double) * ncodes * dim);
     if (!rc)
         rc = dev_alloc(ctx, (void **)&t->d_weights, sizeof(uint32_t) * nsites);
     if (rc) {
         crux_tree_destroy(t);
         return rc;
     }
-    hipError_t e = hipMemcpy(t->d_codes, codes, (size_t)ntaxa * nsites, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(t->d_codevecs, code_vecs, sizeof(double) * ncodes * dim,
-                      hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(t->d_codevecs, code_vecs, sizeof(double) * ncodes * dim,
+                             hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(t->d_weights, weights, sizeof(uint32_t) * nsites, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
@@ -1411,7 +1405,6 @@ int crux_tree_destroy(crux_tree *t)
     (void)hipStreamSynchronize(ctx->stream);
     arena_reset(t, 0);
     arena_reset(t, 1);
-    dev_free(ctx, t->d_codes, (size_t)t->ntaxa * t->S);
     dev_free(ctx, t->d_codevecs, sizeof(double) * t->ncodes * t->K);
     dev_free(ctx, t->d_masks, (size_t)t->ntaxa * t->mask_stride());
     dev_free(ctx, t->d_weights, sizeof(uint32_t) * t->S);
