@@ -129,8 +129,7 @@ struct Entry {
     const double *piw;     // [C][K]
     const uint32_t *weights;
     double *site_out;      // [S] pattern-weighted site lnLs
-    double *partials;      // [ntiles][2] compensated block sums
-    double *lnl_out;       // final lnL
+    double *partials;      // [ntiles][2] compensated per-wave sums (host memory)
     const uint8_t *const *tip_rows;  // padded tip row of each tip child, plan order
     int32_t ntips;
     int32_t pad_;

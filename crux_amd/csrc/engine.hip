@@ -33,7 +33,6 @@ int traverse_cmax(int K, int C);
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
                     int blk0, int nblocks, int depth, hipStream_t stream);
 void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream);
-void launch_finalize(const Entry *entries, int nentries, hipStream_t stream);
 
 static thread_local char g_errbuf[512] = "";
 
@@ -88,17 +87,24 @@ struct crux_ctx {
         size_t plan_cap = 0;
         float *d_pmat = nullptr;       // P blocks and tip tables
         size_t pmat_cap = 0;           // floats
-        double *d_partials = nullptr;  // [sum ntiles][2]
+        // Per-wave (sum, compensation) pairs, [sum ntiles][2], in coherent
+        // pinned host memory: the traversal writes them there and the host
+        // finishes each evaluation's sum once the batch is done.
+        double *h_partials = nullptr;
+        double *d_partials = nullptr;  // the device's address of h_partials
         size_t partials_cap = 0;       // doubles
-        double *h_lnl = nullptr;       // pinned, coherent: the reduction writes it
-        double *d_lnl = nullptr;       // the device's address of h_lnl
-        size_t lnl_cap = 0;
+
         hipEvent_t done = nullptr;
         bool pending = false;
         bool timed = false;
         uint64_t ticket = 0;
         std::vector<crux_lik *> liks;                  // the batch, in call order
-        std::vector<std::pair<crux_lik *, int>> items; // evaluated: (lik, index)
+        struct Done {
+            crux_lik *L;
+            size_t poff;     // first partial pair of the evaluation
+            size_t ntiles;
+        };
+        std::vector<Done> items;                       // evaluated in this batch
     } stg[2];
     int cur = 0;                   // slot the next batch uses
     uint64_t next_ticket = 1;
@@ -708,6 +714,25 @@ static std::vector<int32_t> assign_tips(std::vector<PlanOp> &ops, bool per_launc
 
 using Staging = crux_ctx::Staging;
 
+// Neumaier addition of (x, xc) into (s, c); an infinite running sum drops
+// the compensation so that -inf stays -inf (the kernels' wave sums do the
+// same).
+static void nsum_merge_host(double &s, double &c, double x, double xc)
+{
+    const double t = s + x;
+    if (std::isfinite(t)) {
+        if (std::fabs(s) >= std::fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+    } else {
+        c = 0.0;
+    }
+    s = t;
+    if (std::isfinite(s))
+        c += xc;
+}
+
 // Finish the batch in staging slot k (and first any older batch in the other
 // slot): wait for its event, hand the results to the likelihoods, keep them
 // under the batch's ticket for crux_batch_wait.
@@ -735,9 +760,15 @@ static int complete_slot(crux_ctx *ctx, int k)
         ctx->last_ms[2] = c;
         ctx->last_ms[3] = d;
     }
+    // Each evaluation's lnL: compensated (Neumaier) sum of its waves'
+    // partial pairs in tile order, so the result is the same run to run.
     for (auto &it : S.items) {
-        it.first->last_lnl = S.h_lnl[it.second];
-        it.first->have_lnl = true;
+        double sum = 0.0, cmp = 0.0;
+        for (size_t k = 0; k < it.ntiles; k++)
+            nsum_merge_host(sum, cmp, S.h_partials[2 * (it.poff + k)],
+                            S.h_partials[2 * (it.poff + k) + 1]);
+        it.L->last_lnl = sum + cmp;
+        it.L->have_lnl = true;
     }
     std::vector<double> r(S.liks.size());
     for (size_t i = 0; i < S.liks.size(); i++)
@@ -887,24 +918,18 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     int rc = grow(ctx, &S.d_pmat, &S.pmat_cap, std::max<size_t>(pm_floats, 64));
     if (rc)
         return rc;
-    rc = grow(ctx, &S.d_partials, &S.partials_cap, std::max<size_t>(2 * tiles, 2));
-    if (rc)
-        return rc;
-    if (liks.size() > S.lnl_cap) {
-        size_t ncap = std::max(liks.size(), S.lnl_cap * 2);
-        if (S.h_lnl)
-            (void)hipHostFree(S.h_lnl);
-        S.h_lnl = nullptr;
-        S.d_lnl = nullptr;
-        S.lnl_cap = 0;
-        // Results land in host memory straight from the reduction kernel
-        // (no copy back to enqueue).
-        HIP_TRY(hipHostMalloc((void **)&S.h_lnl, ncap * sizeof(double),
+    if (2 * tiles > S.partials_cap) {
+        const size_t ncap = std::max<size_t>(2 * tiles, 2 * S.partials_cap);
+        if (S.h_partials)
+            (void)hipHostFree(S.h_partials);
+        S.h_partials = nullptr;
+        S.d_partials = nullptr;
+        S.partials_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&S.h_partials, ncap * sizeof(double),
                               hipHostMallocCoherent | hipHostMallocMapped));
-        HIP_TRY(hipHostGetDevicePointer((void **)&S.d_lnl, S.h_lnl, 0));
-        S.lnl_cap = ncap;
+        HIP_TRY(hipHostGetDevicePointer((void **)&S.d_partials, S.h_partials, 0));
+        S.partials_cap = ncap;
     }
-
     char *H = S.h_plan, *D = S.d_plan;
     Entry *hent = (Entry *)(H + off_ent);
     Op *hops = (Op *)(H + off_ops);
@@ -924,7 +949,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         size_t blk0, nblk;
     };
     std::vector<Launch> launches;
-    std::vector<size_t> main_entry(items.size());
+    std::vector<size_t> item_poff(items.size()), item_ntl(items.size());
 
     for (size_t ii = 0; ii < items.size(); ii++) {
         crux_lik *L = items[ii].L;
@@ -1046,7 +1071,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         base.weights = t->d_weights;
         base.site_out = L->d_site;
         base.partials = S.d_partials + 2 * itile;
-        base.lnl_out = S.d_lnl + items[ii].idx;
+        item_poff[ii] = itile;
+        item_ntl[ii] = ntl;
         base.ntips = (int32_t)items[ii].tips.size();
         base.tip_rows = nullptr;
         base.pdense = K == 4 ? S.d_pmat + pdense : nullptr;
@@ -1060,7 +1086,6 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         itile += ntl;
         if (!plain) {
             base.tile0 = (int32_t)iblk;
-            main_entry[ii] = ient;
             hent[ient] = base;
             for (size_t b = 0; b < ntl; b++)
                 hblk[iblk++] = (int32_t)ient;
@@ -1070,7 +1095,6 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
             launches.back().nblk += ntl;
         } else {
             // One launch per node update, each on a one-op entry.
-            main_entry[ii] = ient;
             for (size_t o = 0; o < L->ops.size(); o++) {
                 Entry e1 = base;
                 e1.ops = dops + op0 + o;
@@ -1112,19 +1136,6 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     }
     if (ctx->timing)
         HIP_TRY(hipEventRecord(ctx->ev[2], st));
-    // Finalize: one wave per evaluation, over the evaluations' main entries.
-    // In plain mode the root op's entry carries the partials; the entries of
-    // one evaluation share partials and lnl_out, so finalize the last one.
-    if (!plain) {
-        launch_finalize(dent, (int)ient, st);
-    } else {
-        // Finalize reads ntiles/partials/lnl_out, identical across the
-        // one-op entries of an evaluation: launch over each evaluation's
-        // first entry.
-        for (size_t ii = 0; ii < items.size(); ii++)
-            launch_finalize(dent + main_entry[ii], 1, st);
-    }
-    HIP_TRY(hipGetLastError());
     if (ctx->timing)
         HIP_TRY(hipEventRecord(ctx->ev[3], st));
     HIP_TRY(hipEventRecord(S.done, st));
@@ -1136,8 +1147,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     S.ticket = ctx->next_ticket++;
     S.liks = liks;
     S.items.clear();
-    for (Item &it : items)
-        S.items.push_back({it.L, it.idx});
+    for (size_t ii = 0; ii < items.size(); ii++)
+        S.items.push_back({items[ii].L, item_poff[ii], item_ntl[ii]});
     ctx->cur = slot ^ 1;
     *ticket = S.ticket;
     return CRUX_OK;
@@ -1247,11 +1258,10 @@ int crux_ctx_destroy(crux_ctx *ctx)
     for (Staging &S : ctx->stg) {
         if (S.h_plan)
             (void)hipHostFree(S.h_plan);
-        if (S.h_lnl)
-            (void)hipHostFree(S.h_lnl);
+        if (S.h_partials)
+            (void)hipHostFree(S.h_partials);
         dev_free(ctx, S.d_plan, S.plan_cap);
         dev_free(ctx, S.d_pmat, S.pmat_cap * sizeof(float));
-        dev_free(ctx, S.d_partials, S.partials_cap * sizeof(double));
         if (S.done)
             (void)hipEventDestroy(S.done);
     }

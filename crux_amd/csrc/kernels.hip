@@ -19,7 +19,8 @@
 //              is not stored: it is reduced to the pattern-weighted site lnL
 //              in FP64, NaN -> -inf (CxLik.c:598-632), and to a compensated
 //              per-wave sum.
-//  k_finalize  Compensated FP64 sum of the per-block partials per evaluation.
+//              Each wave writes its compensated sum to host memory; the
+//              host finishes the evaluation's sum once the batch is done.
 
 #include <hip/hip_runtime.h>
 
@@ -995,44 +996,6 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     }
 }
 
-__global__ void __launch_bounds__(64) k_finalize(const Entry *__restrict__ entries)
-{
-    const Entry *E = &entries[blockIdx.x];
-    const int nt = E->ntiles, lane = (int)threadIdx.x;
-    const CRUX_GAS double *pp = gas(E->partials);
-    double s = 0.0, c = 0.0;
-    // Lane-strided over the tiles, eight loads in flight per lane (a zero
-    // pair past the end adds nothing).
-    for (int t0 = 0; t0 < nt; t0 += 8 * 64) {
-        double vs[8], vc[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int t = t0 + k * 64 + lane;
-            vs[k] = t < nt ? pp[2 * t] : 0.0;
-            vc[k] = t < nt ? pp[2 * t + 1] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            nsum_merge(s, c, vs[k], vc[k]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double s2 = __shfl_xor(s, off, 64), c2 = __shfl_xor(c, off, 64);
-        // Both lanes of a pair combine in the same order, so every lane ends
-        // with the same value.
-        if (((int)threadIdx.x & off) == 0)
-            nsum_merge(s, c, s2, c2);
-        else {
-            double a = s2, b = c2;
-            nsum_merge(a, b, s, c);
-            s = a;
-            c = b;
-        }
-    }
-    if (threadIdx.x == 0)
-        *E->lnl_out = s + c;
-}
-
 // ---------------------------------------------------------------------------
 // Launchers.
 
@@ -1115,9 +1078,5 @@ void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream)
     hipLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), shmem, stream, jobs);
 }
 
-void launch_finalize(const Entry *entries, int nentries, hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_finalize, dim3(nentries), dim3(64), 0, stream, entries);
-}
 
 }  // namespace crux
