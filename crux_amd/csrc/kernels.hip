@@ -791,12 +791,14 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 #pragma unroll
         for (int q = 0; q < PQ; ++q)
             pr[q] = bload4(rm, voff, tip ? (q < NI ? q : NI - 1) * 16 : (q < NQ ? q : NQ - 1) * 1024);
-        // The zero block reads as a slot would: lanes 16 B apart, float4 q
-        // a wave further.
+        // Only a child cached in HBM has a CLV to load (the registers keep
+        // whatever they held otherwise; nothing reads them then).
+        if (hbm) {
 #pragma unroll
-        for (int q = 0; q < NI; ++q)
-            hxr[q] = bload4(rh, lane16, q * 1024);
-        her = bload1(re, hbm ? (uint32_t)sc * 4 : 0u, 0);
+            for (int q = 0; q < NI; ++q)
+                hxr[q] = bload4(rh, lane16, q * 1024);
+            her = bload1(re, (uint32_t)sc * 4, 0);
+        }
     };
 
     // y = P_c x for every lane's own (site, component).
