@@ -196,6 +196,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *p)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
 }
 
+// A descriptor whose range ends after nbytes: accesses at or past it read
+// zero and write nothing (buffer range checking), so lanes past the end of
+// an array need no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *p, uint32_t nbytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)nbytes, 0x00020000);
+}
+
 __device__ __forceinline__ crux_f32x4 bload4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
     return __builtin_bit_cast(crux_f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
@@ -611,15 +619,15 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                     stk[pp][c][lane] = yl[c];
                 stke[pp][lane] = el;
             }
-            if (act) {
-                const auto rp = rsrc_of(reinterpret_cast<const void *>(par));
+            // Each component plane is its own range of S sites: lanes past the
+            // last site and components past C (an empty range) store nothing.
+            const char *pp0 = reinterpret_cast<const char *>(par);
 #pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    if (c < C)
-                        bstore4(crux_f32x4{yl[c].x, yl[c].y, yl[c].z, yl[c].w}, rp,
-                                (uint32_t)s * 16, (uint32_t)(c * S) * 16);
-                bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex)), (uint32_t)s * 4, 0);
-            }
+            for (int c = 0; c < CM; ++c)
+                bstore4(crux_f32x4{yl[c].x, yl[c].y, yl[c].z, yl[c].w},
+                        rsrc_of(pp0 + (size_t)c * S * 16, c < C ? (uint32_t)S * 16 : 0u),
+                        (uint32_t)s * 16, 0);
+            bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), (uint32_t)s * 4, 0);
         } else {
             double L = 0.0;
 #pragma unroll
