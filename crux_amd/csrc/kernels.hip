@@ -338,7 +338,10 @@ __device__ __forceinline__ float4 pmul4_mfma(float4 prow, float4 x)
 // children's stack entries and tip masks at once before picking by kind.
 // Tip masks of up to kTipChunk tips are staged when the wave starts (four
 // sites of a row per lane and load); a plan with more tips stages the next
-// chunk where it starts.
+// chunk where it starts.  CLV stores and cached-child loads go through raw
+// buffer descriptors (scalar base and range): the address arithmetic stays
+// on the scalar unit, and lanes past the last site fall outside a store's
+// range instead of branching around it.
 template <int CM>
 __host__ __device__ constexpr int trav4_pchunk()
 {
