@@ -155,8 +155,9 @@ int crux_lik_flush(crux_lik *lik);
 /* ---- batches ------------------------------------------------------------ */
 
 /* Evaluate n likelihoods in one fused device pass (one P-matrix launch, one
- * traversal launch per state-count/component-count class, one reduction
- * launch).  Each lik must be on a different tree-polarity.  out[i] receives
+ * traversal launch per state-count/component-count class; the waves' partial
+ * sums are finished on the host).  Each lik must be on a different
+ * tree-polarity.  out[i] receives
  * lnL of liks[i] rooted at roots[i].  Synchronous.  Replaces, for a batch,
  * the per-Lik CxLikExecute calls of Lik.lnL (CxLik.h:285, Lik.pyx:1699). */
 int crux_batch_lnl(crux_ctx *ctx, int n, crux_lik *const *liks,
