@@ -107,16 +107,19 @@ class _DeviceTree:
         if self.sn != t.tsn:
             rings, nodes = self.tsink
             if self.ids == (t.max_node_id, t.max_edge_id):
+                # A few rings and nodes (an MC3 move): plain ctypes arrays
+                # cost far less than numpy ones at that size.
                 rl = sorted(rings, key=lambda r: r.id)
                 nl = sorted(nodes, key=lambda n: n.id)
-                ids = nat.i32([r.id for r in rl])
-                rnode = nat.i32([r.node.id if r.node is not None else -1 for r in rl])
-                rnext = nat.i32([r.next.id if r.node is not None else -1 for r in rl])
-                nid = nat.i32([n.id for n in nl])
-                nring = nat.i32([n.ring.id if n.ring is not None else -1 for n in nl])
+                nr, nn = len(rl), len(nl)
+                ri = nat.ctypes.c_int32 * nr
+                ni = nat.ctypes.c_int32 * nn
                 nat.check(nat.lib().crux_tree_update_rings(
-                    self.handle, len(ids), nat.iptr(ids), nat.iptr(rnode), nat.iptr(rnext),
-                    len(nid), nat.iptr(nid), nat.iptr(nring)))
+                    self.handle, nr, ri(*[r.id for r in rl]),
+                    ri(*[r.node.id if r.node is not None else -1 for r in rl]),
+                    ri(*[r.next.id if r.node is not None else -1 for r in rl]),
+                    nn, ni(*[n.id for n in nl]),
+                    ni(*[n.ring.id if n.ring is not None else -1 for n in nl])))
             else:
                 rn, rx, nr, nt, el = t.topology_arrays(self.alignment.taxaMap)
                 nat.check(nat.lib().crux_tree_set_topology(
