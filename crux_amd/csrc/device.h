@@ -10,7 +10,8 @@
 //                planes, so a wave's float4 access covers 1 KiB contiguously)
 //   tip rows   : uint8  rows[ntaxa][S64]     K == 4: 4-bit state set; K != 4:
 //                the code index; rows padded to S64 = S rounded up to 64
-//   P block    : float  P[CM][4][4]          K == 4, row = parent state
+//   P image 4  : float  img[p4_regs][64]     K == 4: an update's P matrices as
+//                the matrix core's broadcast A operand (see p4_regs)
 //   P image    : float  img[NVP/4][64][4]    K != 4: P laid out as the matrix
 //                core's A operand (see img_nv)
 //   tip table  : float  T[ncodes][CM][KP]    K != 4: T = P * leaf vector of each
@@ -133,9 +134,9 @@ struct Entry {
     const uint8_t *const *tip_rows;  // padded tip row of each tip child, plan order
     int32_t ntips;
     int32_t pad_;
-    // 4-state: P blocks of the plan, dense [op][child][CM][16] with CM the
-    // component class (rows of components >= C zero; child slots without a P
-    // block are left unused), staged into LDS a chunk of updates at a time.
+    // 4-state: P images of the plan, p4_floats(CM) per update with CM the
+    // component class (columns of child slots without P are left unused),
+    // staged into LDS a chunk of updates at a time.
     const float *pdense;
     // Control words of each update, kCtlWords (4-state) or kCtlWords20 per
     // update: nchild | kinds << 8 | from_stack << 16 | forwarded << 20 (bit
@@ -155,10 +156,18 @@ constexpr int kCtlWords20 = 32;
 constexpr size_t kZeroBytes = 64 * 1024;
 
 enum PLayout : int32_t {
-    kPBlock = 0,   // P[cm][K][K], rows of components >= C zero (4-state)
+    kPImage4 = 0,  // 4-state matrix-core image of one child (see p4_regs)
     kPImage = 1,   // matrix-core image, img_floats(kp, cm) floats
     kPTips = 2,    // tip table T[ncodes][cm][kp]
 };
+
+// 4 states: the P matrices of one update as broadcast A operands of
+// v_mfma_f32_4x4x1f32, p4_regs(cm) registers of 64 lanes ([reg][lane]
+// floats).  Column q = (j * cm + c) * 4 + k (child j, component c, column k)
+// lives in register q / 16, lanes 4 (q % 16) + i holding P_c[i][k]; rows of
+// components c >= C are zero.
+__host__ __device__ constexpr int p4_regs(int cm) { return (kOpMaxChild * cm * 4 + 15) / 16; }
+__host__ __device__ constexpr int p4_floats(int cm) { return p4_regs(cm) * 64; }
 
 struct PJob {
     double len;            // branch length (times comp_rate[c] gives t)
@@ -171,6 +180,7 @@ struct PJob {
     int32_t ncodes;
     int32_t cm;            // component class
     int32_t kp;            // padded state count (K != 4)
+    int32_t child;         // the child's index in its update (4-state image)
 };
 
 __host__ __device__ inline int model_stride(int K) { return 2 * K * K + K + 1; }

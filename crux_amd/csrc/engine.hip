@@ -865,7 +865,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         if (plain)
             nplain += L->ops.size();
         if (K == 4)
-            pm_floats += align_up(L->ops.size() * kOpMaxChild * it.CM * 16, 64);
+            pm_floats += align_up(L->ops.size() * p4_floats(it.CM), 64);
         for (PlanOp &op : L->ops)
             for (PlanChild &c : op.ch) {
                 if (c.kind == kChildIdent)
@@ -961,10 +961,10 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         const size_t ntl = (t->S + tile_sites(K, CM) - 1) / tile_sites(K, CM);
         const size_t op0 = io, ctl0 = ictl;
         const uint64_t clv_b = (uint64_t)t->S * C * KP * 4, exp_b = (uint64_t)t->exp_count(C) * 4;
-        // 4-state: dense [op][child][CM][16] P region of this evaluation.
+        // 4-state: the P images of this evaluation, one per update.
         const size_t pdense = ipm;
         if (K == 4)
-            ipm += align_up(L->ops.size() * kOpMaxChild * CM * 16, 64);
+            ipm += align_up(L->ops.size() * p4_floats(CM), 64);
         for (PlanOp &pop : L->ops) {
             const size_t olocal = io - op0;
             Op &op = hops[io++];
@@ -1039,14 +1039,15 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                     jb.model = L->d_model;
                     jb.codevecs = t->d_codevecs;
                     // The 4-state kernel forms tip products from P itself.
-                    jb.layout = K == 4 ? kPBlock : pc.kind == kChildTip ? kPTips : kPImage;
+                    jb.layout = K == 4 ? kPImage4 : pc.kind == kChildTip ? kPTips : kPImage;
                     jb.C = C;
                     jb.K = K;
                     jb.ncodes = t->ncodes;
                     jb.cm = CM;
                     jb.kp = KP;
+                    jb.child = (int32_t)j;
                     if (K == 4) {
-                        jb.out = S.d_pmat + pdense + (olocal * kOpMaxChild + j) * CM * 16;
+                        jb.out = S.d_pmat + pdense + olocal * p4_floats(CM);
                     } else {
                         jb.out = S.d_pmat + ipm;
                         ipm += align_up(jb.layout == kPTips ? (size_t)t->ncodes * CM * KP
@@ -1100,7 +1101,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
                 e1.ops = dops + op0 + o;
                 e1.nops = 1;
                 if (base.pdense)
-                    e1.pdense = base.pdense + o * kOpMaxChild * CM * 16;
+                    e1.pdense = base.pdense + o * p4_floats(CM);
                 e1.ctl = base.ctl + o * nw;
                 if (base.tip_rows) {
                     e1.tip_rows = base.tip_rows + L->ops[o].tip_first;

@@ -109,6 +109,13 @@ __device__ __forceinline__ void nsum_merge(double &s, double &c, double s2, doub
 // ---------------------------------------------------------------------------
 // P matrices and tip tables.
 
+// Position of P_c[i][k] of child j in a 4-state update image.
+__device__ __forceinline__ int p4_index(int j, int cm, int c, int i, int k)
+{
+    const int q = (j * cm + c) * 4 + k;
+    return (q >> 4) * 64 + (q & 15) * 4 + i;
+}
+
 __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
 {
     extern __shared__ double sm[];
@@ -129,16 +136,17 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
         for (int k = 0; k < K; ++k)
             p += V[i * K + k] * e[k] * Vi[k * K + j];
         p = p < 0.0 ? 0.0 : p;
-        if (J.layout == kPBlock)
-            J.out[idx] = (float)p;
+        if (J.layout == kPImage4)
+            J.out[p4_index(J.child, J.cm, c, i, j)] = (float)p;
         else
             P[idx] = p;
     }
-    if (J.layout == kPBlock) {
-        // 4-state P blocks are strided by the component class: the rows of
-        // the idle components c in [C, cm) are zero.
-        for (int idx = C * KK + (int)threadIdx.x; idx < J.cm * KK; idx += blockDim.x)
-            J.out[idx] = 0.0f;
+    if (J.layout == kPImage4) {
+        // The columns of the idle components c in [C, cm) are zero.
+        for (int idx = C * KK + (int)threadIdx.x; idx < J.cm * KK; idx += blockDim.x) {
+            const int c = idx >> 4, i = (idx >> 2) & 3, k = idx & 3;
+            J.out[p4_index(J.child, J.cm, c, i, k)] = 0.0f;
+        }
         return;
     }
     __syncthreads();
@@ -290,20 +298,13 @@ __device__ __forceinline__ float4 mul4(float4 a, float4 b)
 // y = P x for the wave's 64 sites on the matrix core, FP32 in and out:
 // v_mfma_f32_4x4x1f32 computes 16 independent 4x4 outer products, block b
 // taking the A value of lanes 4b..4b+3 as a column and the B value of the
-// same lanes as a row.  With lane l supplying A = P[l % 4][k] and B = its own
-// x[k], block b accumulates y[i] of site 4b+j in result register i of lane
-// 4b+j, so after k = 0..3 every lane holds P x of its own site.  The sum is
-// the same FMA chain P[i][0] x0 + ... + P[i][3] x3 as on the vector ALU, at
-// twice the FP32 rate and off the vector pipeline.
-__device__ __forceinline__ float4 pmul4_mfma(float4 prow, float4 x)
-{
-    crux_f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
-    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.x, x.x, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.y, x.y, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.z, x.z, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_4x4x1f32(prow.w, x.w, d, 0, 0, 0);
-    return make_float4(d.x, d.y, d.z, d.w);
-}
+// same lanes as a row; with CBSZ 4 every block takes its A column from block
+// ABID instead.  With A = column k of P (one block of an image register,
+// broadcast) and B = each lane's own x[k], block b accumulates y[i] of site
+// 4b+j in result register i of lane 4b+j, so after k = 0..3 every lane holds
+// P x of its own site.  The sum is the same FMA chain P[i][0] x0 + ... +
+// P[i][3] x3 as on the vector ALU, at twice the FP32 rate and off the vector
+// pipeline, and one image register carries 16 columns of P.
 
 // ---------------------------------------------------------------------------
 // 4-state traversal.  A block is one wave: 64 sites, one per lane, with all
@@ -330,10 +331,10 @@ __device__ __forceinline__ float4 pmul4_mfma(float4 prow, float4 x)
 // load issued when an update needs it meets the full L2 latency with nothing
 // to overlap it; and on this GPU one counter tracks loads and stores alike,
 // so a wave that waits for a load also waits for its own CLV stores to
-// drain.  Hence the P blocks and control words are prefetched into registers
+// drain.  Hence the P images and control words are prefetched into registers
 // a chunk of updates ahead and written to the wave's LDS double-buffered
 // (the chunk after next is fetched when a chunk starts and written just
-// before it is needed), the next update's control words and P rows are read
+// before it is needed), the next update's control words and P image are read
 // from LDS while the current one finishes, and an update reads all its
 // children's stack entries and tip masks at once before picking by kind.
 // Tip masks of up to kTipChunk tips are staged when the wave starts (four
@@ -353,7 +354,8 @@ __global__ void __launch_bounds__(kTile4)
 k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
     constexpr int PC = trav4_pchunk<CM>();
-    constexpr int kChunkRows = PC * kOpMaxChild * CM * 4;   // float4 rows of P per chunk
+    constexpr int NR = p4_regs(CM);                 // P image registers per update
+    constexpr int kChunkRows = PC * NR * 16;        // float4s of P per chunk
     __shared__ float4 stk[D][CM][kTile4];
     __shared__ int32_t stke[D][kTile4];
     __shared__ uint32_t tipm[kTipChunk][kTile4 / 4];
@@ -372,7 +374,7 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 
     // ---- P / control-word staging ---------------------------------------
     constexpr int kPer = (kChunkRows + kTile4 - 1) / kTile4;   // float4 per lane
-    const int total_r = nops * kOpMaxChild * CM * 4;
+    const int total_r = nops * NR * 16;
     const CRUX_GAS float4 *pd = gas(reinterpret_cast<const float4 *>(E->pdense));
     const CRUX_GAS int32_t *cg = gas(E->ctl);
     float4 pf[kPer];
@@ -443,27 +445,24 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
     if (ntips > 0)
         stage_tips(0);
 
-    // Control words and P rows (row lane % 4 of every child's and
-    // component's P) of the update about to run.
+    // Control words and P image (one float per lane and register) of the
+    // update about to run.
     int4 cwa, cwb;
-    // Up to four components, an update's P rows and stack entries are read
-    // ahead into registers; with more, at their use (register budget).
+    // Up to four components, an update's stack entries are read ahead into
+    // registers; with more, at their use (register budget).
     constexpr bool kPre = CM <= 4;
-    float4 prow[kOpMaxChild][kPre ? CM : 1];
+    float pr[NR];
     auto read_op = [&](int o) {
         const int rb = (o / PC) & 1, rq = o % PC;
         const int4 *cw = reinterpret_cast<const int4 *>(&cch[rb][rq * kCtlWords]);
         cwa = cw[0];
         cwb = cw[1];
-        // Components c >= C have all-zero P blocks, so every product below
+        // Components c >= C have all-zero P columns, so every product below
         // runs over all CM components without a branch and leaves them 0.
-        if constexpr (kPre) {
+        const float *img = reinterpret_cast<const float *>(&pch[rb][rq * NR * 16]);
 #pragma unroll
-            for (int j = 0; j < kOpMaxChild; ++j)
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    prow[j][c] = pch[rb][((rq * kOpMaxChild + j) * CM + c) * 4 + (lane & 3)];
-        }
+        for (int v = 0; v < NR; ++v)
+            pr[v] = img[v * 64 + lane];
     };
     read_op(0);
 
@@ -516,12 +515,6 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
             se[j] = stke[p & (D - 1)][lane];
             tm[j] = tipm[p & (kTipChunk - 1)][lane >> 2];
         }
-        auto prow_of = [&](int j, int c) -> float4 {
-            if constexpr (kPre)
-                return prow[j][c];
-            else
-                return pch[buf][((oq * kOpMaxChild + j) * CM + c) * 4 + (lane & 3)];
-        };
         auto stack_of = [&](int j, int c) -> float4 {
             if constexpr (kPre)
                 return sx[j][c];
@@ -533,22 +526,24 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
         // acc = (first child ? v : acc * v), v = P_c x_c (matrix core, the
         // components' chains interleaved; control flow is uniform across the
         // wave here, as the matrix instructions need) or x_c (ident child).
-        auto take = [&](int j, auto &&xc, bool mat) {
+        // Column k of P_c comes from block q % 16 of image register q / 16,
+        // broadcast to all 16 blocks (CBSZ 4, ABID q % 16), so block b
+        // forms P_c[:][k] x_k for its four sites.
+        auto take = [&](auto Jc, auto &&xc, bool mat) {
+            constexpr int j = decltype(Jc)::value;
             if (mat) {
                 crux_f32x4 d[CM];
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).x, xc(c).x,
-                                                              crux_f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).y, xc(c).y, d[c], 0, 0, 0);
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).z, xc(c).z, d[c], 0, 0, 0);
-#pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(prow_of(j, c).w, xc(c).w, d[c], 0, 0, 0);
+                static_for<4>([&](auto Kc) {
+                    constexpr int k = decltype(Kc)::value;
+                    static_for<CM>([&](auto Cc) {
+                        constexpr int c = decltype(Cc)::value;
+                        constexpr int q = (j * CM + c) * 4 + k;
+                        const float4 xv = xc(c);
+                        const float xk = k == 0 ? xv.x : k == 1 ? xv.y : k == 2 ? xv.z : xv.w;
+                        d[c] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+                            pr[q >> 4], xk, k == 0 ? crux_f32x4{0.f, 0.f, 0.f, 0.f} : d[c], 4, q & 15, 0);
+                    });
+                });
 #pragma unroll
                 for (int c = 0; c < CM; ++c) {
                     const float4 v = make_float4(d[c].x, d[c].y, d[c].z, d[c].w);
@@ -560,22 +555,22 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                     acc[c] = (j == 0) ? xc(c) : mul4(acc[c], xc(c));
             }
         };
-#pragma unroll
-        for (int j = 0; j < kOpMaxChild; ++j) {
+        static_for<kOpMaxChild>([&](auto Jc) {
+            constexpr int j = decltype(Jc)::value;
             if (j >= nch)
-                break;
+                return;
             const int kind = (kinds >> (2 * j)) & 3;
             const bool mat = kind != kChildIdent;
             if (kind == kChildTip) {
                 const uint32_t m = tm[j] >> (8 * (lane & 3));
                 const float4 t = make_float4((float)(m & 1u), (float)((m >> 1) & 1u),
                                              (float)((m >> 2) & 1u), (float)((m >> 3) & 1u));
-                take(j, [&](int) { return t; }, true);
+                take(Jc, [&](int) { return t; }, true);
             } else if ((fwd >> j) & 1) {
-                take(j, [&](int c) { return yl[c]; }, mat);
+                take(Jc, [&](int c) { return yl[c]; }, mat);
                 ex += el;
             } else if ((fst >> j) & 1) {
-                take(j, [&](int c) { return stack_of(j, c); }, mat);
+                take(Jc, [&](int c) { return stack_of(j, c); }, mat);
                 ex += se[j];
             } else {
                 // Cached by an earlier plan (partial re-evaluation).  The wait
@@ -591,10 +586,10 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                 }
                 ex += bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
                 __builtin_amdgcn_s_waitcnt(0);
-                take(j, [&](int c) { return x[c]; }, mat);
+                take(Jc, [&](int c) { return x[c]; }, mat);
             }
-        }
-        // The next update's control words and P rows, while this one ends.
+        });
+        // The next update's control words and P image, while this one ends.
         if (o + 1 < nops)
             read_op(o + 1);
         if (has_parent) {
