@@ -163,10 +163,25 @@ enum PLayout : int32_t {
 
 // 4 states: the P matrices of one update as broadcast A operands of
 // v_mfma_f32_4x4x1f32, p4_regs(cm) registers of 64 lanes ([reg][lane]
-// floats).  Column q = (j * cm + c) * 4 + k (child j, component c, column k)
-// lives in register q / 16, lanes 4 (q % 16) + i holding P_c[i][k]; rows of
-// components c >= C are zero.
-__host__ __device__ constexpr int p4_regs(int cm) { return (kOpMaxChild * cm * 4 + 15) / 16; }
+// floats); rows of components c >= C are zero.  cm == 1 (a lane per site):
+// column q = j * 4 + k (child j, column k) lives in register q / 16, lanes
+// 4 (q % 16) + i holding P[i][k].  cm >= 2 (lane half h holds components
+// h * cm/2 .. h * cm/2 + cm/2 - 1): column q = (j * cm/2 + m) * 4 + k of
+// component c = h * cm/2 + m lives in register q / 8, lanes
+// 32 h + 4 (q % 8) + i.
+__host__ __device__ constexpr int p4_regs(int cm)
+{
+    return cm == 1 ? (kOpMaxChild * 4 + 15) / 16 : (kOpMaxChild * (cm / 2) * 4 + 7) / 8;
+}
+__host__ __device__ constexpr int p4_index(int j, int cm, int c, int i, int k)
+{
+    if (cm == 1) {
+        const int q = j * 4 + k;
+        return (q >> 4) * 64 + (q & 15) * 4 + i;
+    }
+    const int cl = cm / 2, h = c / cl, q = (j * cl + c % cl) * 4 + k;
+    return (q >> 3) * 64 + h * 32 + (q & 7) * 4 + i;
+}
 __host__ __device__ constexpr int p4_floats(int cm) { return p4_regs(cm) * 64; }
 
 struct PJob {

@@ -109,13 +109,6 @@ __device__ __forceinline__ void nsum_merge(double &s, double &c, double s2, doub
 // ---------------------------------------------------------------------------
 // P matrices and tip tables.
 
-// Position of P_c[i][k] of child j in a 4-state update image.
-__device__ __forceinline__ int p4_index(int j, int cm, int c, int i, int k)
-{
-    const int q = (j * cm + c) * 4 + k;
-    return (q >> 4) * 64 + (q & 15) * 4 + i;
-}
-
 __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
 {
     extern __shared__ double sm[];
@@ -667,6 +660,360 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 }
 
 // ---------------------------------------------------------------------------
+// 4-state traversal with the components split over lane halves (CM >= 2).
+// A block is two waves covering one 64-site tile, 32 sites each: lane l of
+// wave w holds site 32 w + l % 32 of the tile and, in half h = l / 32,
+// components h CM/2 .. h CM/2 + CM/2 - 1.  Each lane does half of k_trav4's
+// per-site work and there are twice the waves, two per SIMD, so that one
+// wave issues while the other waits.  The two waves share the staged P images
+// and control words (one barrier per chunk of updates) and keep their own
+// LDS stacks and tip masks.  P x runs on the matrix core with the A operand
+// broadcast within each half (CBSZ 3: the blocks of lanes 0-31 take block
+// ABID, those of lanes 32-63 block 8 + ABID), so an image register carries 8
+// columns of each half's components; the site maximum pairs the halves with
+// v_permlane32_swap.  Plan walk, forwarding, stack, rescaling, stores and
+// the root reduction follow k_trav4; at the root the halves' component sums
+// are added, and the block's two wave sums are merged in site order.
+template <int CM, int D>
+__global__ void __launch_bounds__(2 * kTile4)
+k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
+{
+    static_assert(CM >= 2, "at least one component per lane half");
+    constexpr int CL = CM / 2;                    // components per lane
+    constexpr int HS = kTile4 / 2;                // sites per wave
+    constexpr int PC = trav4_pchunk<CM>();
+    constexpr int NR = p4_regs(CM);               // P image registers per update
+    constexpr int kChunkRows = PC * NR * 16;      // float4s of P per chunk
+    __shared__ float4 stk[2][D][CL][kTile4];
+    __shared__ int32_t stke[2][D][kTile4];
+    __shared__ uint32_t tipm[2][kTipChunk][HS / 4];
+    __shared__ float4 pch[2][kChunkRows];
+    __shared__ int32_t cch[2][PC * kCtlWords];
+    __shared__ double red[2];
+    const int b = blk0 + (int)blockIdx.x;
+    const CRUX_CAS Entry *E = cas(entries) + cas(block_entry)[b];
+    const int S = E->S, C = E->C, nops = E->nops, ntips = E->ntips;
+    const int tile = b - E->tile0;
+    const int tid = (int)threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int h = lane >> 5;
+    const int s = tile * kTile4 + wv * HS + (lane & 31);
+    const bool act = s < S;
+    const int sc = act ? s : S - 1;       // in-bounds site for loads of idle lanes
+    const CRUX_CAS Op *ops = cas(E->ops);
+    double sum = 0.0;
+    // Byte offsets of the lane's site in the first plane of its half; lanes
+    // past the last site store at an offset no buffer range covers.
+    constexpr uint32_t kOut = 0xfffffff0u;
+    const uint32_t vo = act ? (uint32_t)(h * CL * S + s) * 16 : kOut;
+    const uint32_t vol = (uint32_t)(h * CL * S + sc) * 16;
+    const uint32_t voe = (act && h == 0) ? (uint32_t)s * 4 : kOut;
+
+    // Both waves of the block order their LDS writes before a barrier; no
+    // global memory is ordered by it.
+    auto lds_barrier = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+
+    // ---- P / control-word staging, shared by the two waves ---------------
+    constexpr int kPer = (kChunkRows + 2 * kTile4 - 1) / (2 * kTile4);
+    const int total_r = nops * NR * 16;
+    const CRUX_GAS float4 *pd = gas(reinterpret_cast<const float4 *>(E->pdense));
+    const CRUX_GAS int32_t *cg = gas(E->ctl);
+    float4 pf[kPer];
+    int32_t cf = 0;
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            // Rows past the chunk or the plan load a real row (never read).
+            const int i = min(q * 2 * kTile4 + tid, kChunkRows - 1);
+            const int g = min(k * kChunkRows + i, total_r - 1);
+            const crux_f32x4 v = *(const CRUX_GAS crux_f32x4 *)(pd + g);
+            pf[q] = make_float4(v.x, v.y, v.z, v.w);
+        }
+        if (tid < PC * kCtlWords)
+            cf = (k * PC + tid / kCtlWords < nops) ? cg[k * PC * kCtlWords + tid] : 0;
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = q * 2 * kTile4 + tid;
+            if (i < kChunkRows)
+                pch[buf][i] = pf[q];
+        }
+        if (tid < PC * kCtlWords)
+            cch[buf][tid] = cf;
+        lds_barrier();
+    };
+
+    // ---- tip staging: rows [base, base + n) of the entry's tip list --------
+    // One dword covers four sites of a row: 8 rows of the wave's 32 sites
+    // per load instruction.  Mask rows are padded to whole 64-site tiles.
+    auto stage_tips = [&](int base) {
+        constexpr int kDw = HS / 4;                  // dwords of a row per wave
+        constexpr int kRows = 64 / kDw;              // rows per instruction
+        constexpr int kInstr = kTipChunk / kRows;
+        const int n = min(kTipChunk, ntips - base);
+        const CRUX_GAS uint8_t *const CRUX_GAS *rows =
+            (const CRUX_GAS uint8_t *const CRUX_GAS *)gas(E->tip_rows) + base;
+        const int sub = lane / kDw, dw = lane % kDw;
+        const int off = tile * kTile4 + wv * HS + dw * 4;
+        uint32_t v[kInstr];
+#pragma unroll
+        for (int i = 0; i < kInstr; ++i) {
+            const int r = kRows * i + sub;
+            v[i] = 0u;
+            if (kRows * i < n) {
+                const CRUX_GAS uint8_t *p = rows[min(r, n - 1)];
+                if (r < n)
+                    v[i] = *(const CRUX_GAS uint32_t *)(p + off);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kInstr; ++i) {
+            const int r = kRows * i + sub;
+            if (r < n)
+                tipm[wv][r][dw] = v[i];
+        }
+        // No load may stay outstanding past this point: the compiler would
+        // otherwise make later updates wait for their stores as well.
+        __builtin_amdgcn_s_waitcnt(0);
+    };
+
+    fetch(0);
+    commit(0);
+    if (nops > PC)
+        fetch(1);
+    if (ntips > 0)
+        stage_tips(0);
+
+    // Control words and P image of the update about to run.
+    int4 cwa, cwb;
+    float pr[NR];
+    auto read_op = [&](int o) {
+        const int rb = (o / PC) & 1, rq = o % PC;
+        const int4 *cw = reinterpret_cast<const int4 *>(&cch[rb][rq * kCtlWords]);
+        cwa = cw[0];
+        cwb = cw[1];
+        const float *img = reinterpret_cast<const float *>(&pch[rb][rq * NR * 16]);
+#pragma unroll
+        for (int v = 0; v < NR; ++v)
+            pr[v] = img[v * 64 + lane];
+    };
+    read_op(0);
+
+    float4 yl[CL];
+    int32_t el = 0;
+#pragma unroll
+    for (int m = 0; m < CL; ++m)
+        yl[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    for (int o = 0; o < nops; ++o) {
+        const CRUX_CAS Op *op = ops + o;
+        const int buf = (o / PC) & 1, oq = o % PC;
+        if (oq == 0 && o > 0 && o + PC < nops)
+            fetch(o / PC + 1);
+        const int w0 = __builtin_amdgcn_readfirstlane(cwa.x);
+        const int32_t pk = __builtin_amdgcn_readfirstlane(cwa.y);
+        const int tl = __builtin_amdgcn_readfirstlane(cwa.z);
+        const bool has_parent = __builtin_amdgcn_readfirstlane(cwa.w) != 0;
+        const uint64_t par = (uint32_t)__builtin_amdgcn_readfirstlane(cwb.x) |
+                             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cwb.y) << 32);
+        const uint64_t pex = (uint32_t)__builtin_amdgcn_readfirstlane(cwb.z) |
+                             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cwb.w) << 32);
+        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xf,
+                  fwd = (w0 >> 20) & 0xf;
+        if (tl > 0)
+            stage_tips(tl);
+        if (oq == PC - 1 && o + 1 < nops)
+            commit(buf ^ 1);
+
+        float4 sx[kOpMaxChild][CL];
+        int32_t se[kOpMaxChild];
+        uint32_t tm[kOpMaxChild];
+#pragma unroll
+        for (int j = 0; j < kOpMaxChild; ++j) {
+            const int p = op_pos(pk, j);
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                sx[j][m] = stk[wv][p & (D - 1)][m][lane];
+            se[j] = stke[wv][p & (D - 1)][lane];
+            tm[j] = tipm[wv][p & (kTipChunk - 1)][(lane & 31) >> 2];
+        }
+        float4 acc[CL];
+        int32_t ex = 0;
+        auto take = [&](auto Jc, auto &&xc, bool mat) {
+            constexpr int j = decltype(Jc)::value;
+            if (mat) {
+                crux_f32x4 d[CL];
+                static_for<4>([&](auto Kc) {
+                    constexpr int k = decltype(Kc)::value;
+                    static_for<CL>([&](auto Mc) {
+                        constexpr int m = decltype(Mc)::value;
+                        constexpr int q = (j * CL + m) * 4 + k;
+                        const float4 xv = xc(m);
+                        const float xk = k == 0 ? xv.x : k == 1 ? xv.y : k == 2 ? xv.z : xv.w;
+                        d[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+                            pr[q >> 3], xk, k == 0 ? crux_f32x4{0.f, 0.f, 0.f, 0.f} : d[m], 3, q & 7, 0);
+                    });
+                });
+#pragma unroll
+                for (int m = 0; m < CL; ++m) {
+                    const float4 v = make_float4(d[m].x, d[m].y, d[m].z, d[m].w);
+                    acc[m] = (j == 0) ? v : mul4(acc[m], v);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    acc[m] = (j == 0) ? xc(m) : mul4(acc[m], xc(m));
+            }
+        };
+        static_for<kOpMaxChild>([&](auto Jc) {
+            constexpr int j = decltype(Jc)::value;
+            if (j >= nch)
+                return;
+            const int kind = (kinds >> (2 * j)) & 3;
+            const bool mat = kind != kChildIdent;
+            if (kind == kChildTip) {
+                const uint32_t mk = tm[j] >> (8 * (lane & 3));
+                const float4 t = make_float4((float)(mk & 1u), (float)((mk >> 1) & 1u),
+                                             (float)((mk >> 2) & 1u), (float)((mk >> 3) & 1u));
+                take(Jc, [&](int) { return t; }, true);
+            } else if ((fwd >> j) & 1) {
+                take(Jc, [&](int m) { return yl[m]; }, mat);
+                ex += el;
+            } else if ((fst >> j) & 1) {
+                take(Jc, [&](int m) { return sx[j][m]; }, mat);
+                ex += se[j];
+            } else {
+                // Cached by an earlier plan (partial re-evaluation).  Planes
+                // past the slot's C read as zero (buffer range).  The wait
+                // stays inside this branch (see stage_tips).
+                const CRUX_CAS OpChild &ch = op->ch[j];
+                const char *cb = reinterpret_cast<const char *>(ch.clv);
+                float4 x[CL];
+#pragma unroll
+                for (int m = 0; m < CL; ++m) {
+                    const crux_f32x4 v =
+                        bload4(rsrc_of(cb + (size_t)m * S * 16, (uint32_t)(C - m) * S * 16), vol, 0);
+                    x[m] = make_float4(v.x, v.y, v.z, v.w);
+                }
+                ex += bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
+                __builtin_amdgcn_s_waitcnt(0);
+                take(Jc, [&](int m) { return x[m]; }, mat);
+            }
+        });
+        // The next update's control words and P image, while this one ends.
+        if (o + 1 < nops)
+            read_op(o + 1);
+        if (has_parent) {
+            // Site maximum over the lane's components, then over the halves.
+            int32_t mb = 0;
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                mb = max(max(mb, max(__float_as_int(acc[m].x), __float_as_int(acc[m].y))),
+                         max(__float_as_int(acc[m].z), __float_as_int(acc[m].w)));
+            {
+                uint32_t a0, a1;
+                lane_pair<32>((uint32_t)mb, a0, a1);
+                mb = max((int32_t)a0, (int32_t)a1);
+            }
+            const float mx = __int_as_float(mb);
+            int e2 = 0;
+            if (mx > 0.0f)
+                (void)frexpf(mx, &e2);
+            el = ex + e2;
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                yl[m] = make_float4(ldexpf(acc[m].x, -e2), ldexpf(acc[m].y, -e2),
+                                    ldexpf(acc[m].z, -e2), ldexpf(acc[m].w, -e2));
+            const int pp = op_pos(pk, kOpMaxChild);
+            if (pp >= 0) {
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    stk[wv][pp][m][lane] = yl[m];
+                stke[wv][pp][lane] = el;
+            }
+            // Store m of half h writes plane h CL + m: the range of store m
+            // reaches the second half's plane only where that plane exists.
+            const char *pp0 = reinterpret_cast<const char *>(par);
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                bstore4(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w},
+                        rsrc_of(pp0 + (size_t)m * S * 16,
+                                (CL + m < C ? (uint32_t)(CL + 1) : 1u) * (uint32_t)S * 16),
+                        vo, 0);
+            bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), voe, 0);
+        } else {
+            double L = 0.0;
+#pragma unroll
+            for (int m = 0; m < CL; ++m) {
+                const CRUX_CAS double *w0p = cas(E->piw) + 4 * m;
+                double wx = w0p[0], wy = w0p[1], wz = w0p[2], wq = w0p[3];
+                if (CL + m < C) {
+                    const CRUX_CAS double *w1p = cas(E->piw) + 4 * (CL + m);
+                    if (h) {
+                        wx = w1p[0];
+                        wy = w1p[1];
+                        wz = w1p[2];
+                        wq = w1p[3];
+                    }
+                } else if (h) {
+                    wx = wy = wz = wq = 0.0;
+                }
+                L += wx * (double)acc[m].x + wy * (double)acc[m].y +
+                     wz * (double)acc[m].z + wq * (double)acc[m].w;
+            }
+            {
+                const uint64_t u = (uint64_t)__double_as_longlong(L);
+                uint32_t al, bl, ah, bh;
+                lane_pair<32>((uint32_t)u, al, bl);
+                lane_pair<32>((uint32_t)(u >> 32), ah, bh);
+                // Commutative: both halves get the same sum.
+                L = __longlong_as_double((long long)(((uint64_t)ah << 32) | al)) +
+                    __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
+            }
+            double v = (log(L) + (double)ex * 0.69314718055994530942) *
+                       (double)gas(E->weights)[sc];
+            if (__builtin_isnan(v))
+                v = -INFINITY;
+            if (act && h == 0) {
+                gas(E->site_out)[s] = v;
+                sum = v;
+            }
+        }
+    }
+    // Compensated wave reduction: both lanes of a pair merge in the same order.
+    double cmp = 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double s2 = __shfl_xor(sum, off, 64), c2 = __shfl_xor(cmp, off, 64);
+        if ((lane & off) == 0) {
+            nsum_merge(sum, cmp, s2, c2);
+        } else {
+            double a = s2, bb = c2;
+            nsum_merge(a, bb, sum, cmp);
+            sum = a;
+            cmp = bb;
+        }
+    }
+    // The tile's sum: wave 0's sites, then wave 1's.
+    if (wv == 1 && lane == 0) {
+        red[0] = sum;
+        red[1] = cmp;
+    }
+    lds_barrier();
+    if (wv == 0 && lane == 0) {
+        nsum_merge(sum, cmp, red[0], red[1]);
+        gas(E->partials)[2 * tile] = sum;
+        gas(E->partials)[2 * tile + 1] = cmp;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // 20/22-state traversal.  A block is one wave covering CM mixture components
 // of G = 64 / CM consecutive sites: lane l holds component l / G of site
 // l % G, all KP states of it in registers (states >= K stay 0).  Like the
@@ -1011,8 +1358,13 @@ template <int CM, int D>
 static void launch_trav4(const Entry *entries, const int32_t *block_entry, int blk0,
                          int nblocks, hipStream_t stream)
 {
-    if constexpr (D <= stack_depth4_max(CM))
+    if constexpr (D > stack_depth4_max(CM))
+        return;
+    else if constexpr (CM == 1)
         hipLaunchKernelGGL((k_trav4<CM, D>), dim3(nblocks), dim3(kTile4), 0, stream,
+                           entries, block_entry, blk0);
+    else
+        hipLaunchKernelGGL((k_trav4h<CM, D>), dim3(nblocks), dim3(2 * kTile4), 0, stream,
                            entries, block_entry, blk0);
 }
 

@@ -94,22 +94,26 @@ def test_random_trees_vs_oracle(ntaxa, nsites, seed, kind):
         assert close(a, b), (i, a, b)
 
 
-@pytest.mark.parametrize("ncat,kind", [(1, "aa"), (2, "aa"), (3, "aa"), (6, "aa"), (1, "protein"),
+@pytest.mark.parametrize("ncat,kind", [(1, "dna"), (2, "dna"), (3, "dna"), (6, "dna"), (12, "dna"),
+                                       (1, "aa"), (2, "aa"), (3, "aa"), (6, "aa"), (1, "protein"),
                                        (2, "protein"), (6, "protein")])
 def test_state_and_component_classes(ncat, kind):
-    # Every component class of the 20/22-state kernel (1, 2, 4 with an idle
-    # component, 8), with every node as root: the roots after the first
-    # re-evaluate from cached CLVs read back from HBM.
+    # Every component class of the 4-state kernels (1, 2, 4 with an idle
+    # component, 8 with two idle, 16 with four) and of the 20/22-state kernel
+    # (1, 2, 4 with an idle component, 8), with every other node as root: the
+    # roots after the first re-evaluate from cached CLVs read back from HBM.
     say("classes", ncat, kind)
     rng = np.random.default_rng(60 + ncat)
-    if kind == "aa":
-        prob = synth.make_problem(18, 600, 60 + ncat, kind="aa", ncat=ncat)
+    if kind in ("dna", "aa"):
+        n = 70 if kind == "dna" else 18
+        prob = synth.make_problem(n, 600, 60 + ncat, kind=kind, ncat=ncat)
     else:
-        rooted, tree = synth.yule_tree(14, rng, 0.15)
+        # From Newick, so that both trees list their edges in one order.
+        rooted, tree, nw = synth.yule_tree(14, rng, 0.15, newick=True)
         m = synth.Model(rng.lognormal(0, 1, 231), rng.dirichlet(np.ones(22)), 0.6, ncat, Protein)
         aln = synth.simulate(rooted, m, 500, rng)
         aln.compact()
-        prob = synth.Problem(tree, rooted, aln, m, 500, "")
+        prob = synth.Problem(tree, rooted, aln, m, 500, nw)
     dev, ora = _pair(prob, ncat=ncat)
     a, b = dev.lnL(), ora.lnL()
     assert close(a, b), (a, b)
@@ -117,6 +121,12 @@ def test_state_and_component_classes(ncat, kind):
     assert np.allclose(sd, so, rtol=1e-6, atol=1e-5)
     for node in list(dev.tree.nodes)[::2]:
         assert close(dev.lnL(node), b), node
+    # One branch changed: the path to it is recomputed, the rest read back.
+    e_d, e_o = list(dev.tree.edges)[7], list(ora.tree.edges)[7]
+    e_d.length *= 1.7
+    e_o.length *= 1.7
+    a, b = dev.lnL(e_d.ring.node), ora.lnL(e_o.ring.node)
+    assert close(a, b), (a, b)
 
 
 def test_aa_many_tips():
