@@ -133,14 +133,15 @@ struct Entry {
     double *partials;      // [ntiles][2] compensated per-wave sums (host memory)
     const uint8_t *const *tip_rows;  // padded tip row of each tip child, plan order
     int32_t ntips;
-    int32_t pad_;
+    int32_t exp_off;       // bytes from a CLV slot to its exponents
     // 4-state: P images of the plan, p4_floats(CM) per update with CM the
     // component class (columns of child slots without P are left unused),
     // staged into LDS a chunk of updates at a time.
     const float *pdense;
     // Control words of each update, kCtlWords (4-state) or kCtlWords20 per
     // update: nchild | kinds << 8 | from_stack << 16 | forwarded << 20 (bit
-    // j: child j is the previous update's result, kept in registers), pos,
+    // j: child j is the previous update's result, kept in registers) |
+    // has_parent << 24, pos,
     // tip_load, has_parent, parent (2 words), parent_exp (2 words); for
     // K != 4 then per child j at 8 + 6 j: P image or tip table, CLV, exponents
     // (2 words each; the context's zero block where the child has none).  The

@@ -316,6 +316,12 @@ static int arena_layout(crux_tree *t, int pol, int C)
     }
     a.slot_bytes = a.clv_bytes + align_up(t->exp_count(C) * sizeof(int32_t), 256);
     a.chunk_slots = 0;
+    // The kernels address a slot with 32-bit byte offsets.
+    if (a.slot_bytes > (size_t)INT32_MAX) {
+        a.C = 0;
+        return set_error(CRUX_EINVAL, "CLV slot of %zu bytes: patterns x components too large",
+                         a.slot_bytes);
+    }
     return CRUX_OK;
 }
 
@@ -532,7 +538,8 @@ static int plan_lik(crux_lik *L, int root)
     // Another lik of this polarity owned the CLVs: everything is stale.
     bool inval = L->invalidate || t->owner[L->pol] != L;
     if (t->arena[L->pol].C != L->C) {
-        arena_layout(t, L->pol, L->C);
+        if (int rc = arena_layout(t, L->pol, L->C))
+            return rc;
         L->root_tmp_slot = -1;
         inval = true;
     }
@@ -985,7 +992,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
             };
             auto write_ctl = [&]() {
                 memset(cw, 0, nw * sizeof(int32_t));
-                cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20);
+                cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20) |
+                        ((pop.parent_slot >= 0 ? 1 : 0) << 24);
                 cw[1] = op.pos;
                 cw[2] = op.tip_load;
                 cw[3] = pop.parent_slot >= 0 ? 1 : 0;
@@ -1075,6 +1083,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         item_poff[ii] = itile;
         item_ntl[ii] = ntl;
         base.ntips = (int32_t)items[ii].tips.size();
+        base.exp_off = (int32_t)A.clv_bytes;
         base.tip_rows = nullptr;
         base.pdense = K == 4 ? S.d_pmat + pdense : nullptr;
         if (!items[ii].tips.empty()) {

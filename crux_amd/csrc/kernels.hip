@@ -741,7 +741,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int i = q * 2 * kTile4 + tid;
-            if (i < kChunkRows)
+            if (kChunkRows % (2 * kTile4) == 0 || i < kChunkRows)
                 pch[buf][i] = pf[q];
         }
         if (tid < PC * kCtlWords)
@@ -761,17 +761,16 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             (const CRUX_GAS uint8_t *const CRUX_GAS *)gas(E->tip_rows) + base;
         const int sub = lane / kDw, dw = lane % kDw;
         const int off = tile * kTile4 + wv * HS + dw * 4;
+        // Row pointers first, then the rows (rows past the list read the
+        // last row again and are not kept): two waits in all.
+        const CRUX_GAS uint8_t *rp[kInstr];
+#pragma unroll
+        for (int i = 0; i < kInstr; ++i)
+            rp[i] = rows[min(kRows * i + sub, n - 1)];
         uint32_t v[kInstr];
 #pragma unroll
-        for (int i = 0; i < kInstr; ++i) {
-            const int r = kRows * i + sub;
-            v[i] = 0u;
-            if (kRows * i < n) {
-                const CRUX_GAS uint8_t *p = rows[min(r, n - 1)];
-                if (r < n)
-                    v[i] = *(const CRUX_GAS uint32_t *)(p + off);
-            }
-        }
+        for (int i = 0; i < kInstr; ++i)
+            v[i] = *(const CRUX_GAS uint32_t *)(rp[i] + off);
 #pragma unroll
         for (int i = 0; i < kInstr; ++i) {
             const int r = kRows * i + sub;
@@ -783,9 +782,13 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         __builtin_amdgcn_s_waitcnt(0);
     };
 
+    // Updates run in chunks of PC: chunk k's P images and control words are
+    // fetched into registers while chunk k - 1 runs and written to LDS
+    // buffer k % 2 when it ends.
+    const int nchunks = (nops + PC - 1) / PC;
     fetch(0);
     commit(0);
-    if (nops > PC)
+    if (nchunks > 1)
         fetch(1);
     if (ntips > 0)
         stage_tips(0);
@@ -793,8 +796,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     // Control words and P image of the update about to run.
     int4 cwa, cwb;
     float pr[NR];
-    auto read_op = [&](int o) {
-        const int rb = (o / PC) & 1, rq = o % PC;
+    auto read_op = [&](int rb, int rq) {
         const int4 *cw = reinterpret_cast<const int4 *>(&cch[rb][rq * kCtlWords]);
         cwa = cw[0];
         cwb = cw[1];
@@ -803,7 +805,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         for (int v = 0; v < NR; ++v)
             pr[v] = img[v * 64 + lane];
     };
-    read_op(0);
+    read_op(0, 0);
 
     float4 yl[CL];
     int32_t el = 0;
@@ -811,25 +813,24 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     for (int m = 0; m < CL; ++m)
         yl[m] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-    for (int o = 0; o < nops; ++o) {
-        const CRUX_CAS Op *op = ops + o;
-        const int buf = (o / PC) & 1, oq = o % PC;
-        if (oq == 0 && o > 0 && o + PC < nops)
-            fetch(o / PC + 1);
+    for (int k = 0; k < nchunks; ++k) {
+    const int buf = k & 1, o0 = k * PC, n = min(PC, nops - o0);
+    if (k > 0 && k + 1 < nchunks)
+        fetch(k + 1);
+    for (int oq = 0; oq < n; ++oq) {
+        const CRUX_CAS Op *op = ops + o0 + oq;
         const int w0 = __builtin_amdgcn_readfirstlane(cwa.x);
         const int32_t pk = __builtin_amdgcn_readfirstlane(cwa.y);
         const int tl = __builtin_amdgcn_readfirstlane(cwa.z);
-        const bool has_parent = __builtin_amdgcn_readfirstlane(cwa.w) != 0;
         const uint64_t par = (uint32_t)__builtin_amdgcn_readfirstlane(cwb.x) |
                              ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cwb.y) << 32);
-        const uint64_t pex = (uint32_t)__builtin_amdgcn_readfirstlane(cwb.z) |
-                             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cwb.w) << 32);
+        // The exponents follow the CLV planes in the parent's slot.
+        const uint64_t pex = par + (uint32_t)E->exp_off;
+        const bool has_parent = (w0 >> 24) & 1;
         const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xf,
                   fwd = (w0 >> 20) & 0xf;
         if (tl > 0)
             stage_tips(tl);
-        if (oq == PC - 1 && o + 1 < nops)
-            commit(buf ^ 1);
 
         float4 sx[kOpMaxChild][CL];
         int32_t se[kOpMaxChild];
@@ -907,8 +908,8 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             }
         });
         // The next update's control words and P image, while this one ends.
-        if (o + 1 < nops)
-            read_op(o + 1);
+        if (oq + 1 < n)
+            read_op(buf, oq + 1);
         if (has_parent) {
             // Site maximum over the lane's components, then over the halves.
             int32_t mb = 0;
@@ -985,6 +986,13 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 sum = v;
             }
         }
+    }
+    // The next chunk goes into the other buffer; its first update is read
+    // after the barrier.
+    if (k + 1 < nchunks) {
+        commit(buf ^ 1);
+        read_op(buf ^ 1, 0);
+    }
     }
     // Compensated wave reduction: both lanes of a pair merge in the same order.
     double cmp = 0.0;
