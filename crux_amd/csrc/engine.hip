@@ -525,6 +525,28 @@ struct Planner {
 
 }  // namespace
 
+// Shape of a 4-state update for the traversal's specialised bodies: 1 +
+// 4 s0 + s1 for two children from sources s0, s1 (ChildSrc) under a parent
+// with no identity child, else 0.
+static int op_shape4(const Op &op, const PlanOp &pop)
+{
+    if (op.nchild != 2 || pop.parent_slot < 0)
+        return 0;
+    int src[2];
+    for (int j = 0; j < 2; j++) {
+        const int kind = (op.kinds >> (2 * j)) & 3;
+        if (kind == kChildIdent)
+            return 0;
+        src[j] = kind == kChildTip ? kSrcTip
+                 : ((pop.fwd >> j) & 1) ? kSrcFwd
+                 : ((op.from_stack >> j) & 1) ? kSrcStk
+                                              : kSrcHbm;
+    }
+    if (src[0] == kSrcFwd && src[1] == kSrcFwd)
+        return 0;
+    return 1 + 4 * src[0] + src[1];
+}
+
 static int plan_lik(crux_lik *L, int root)
 {
     crux_tree *t = L->tree;
@@ -993,7 +1015,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
             auto write_ctl = [&]() {
                 memset(cw, 0, nw * sizeof(int32_t));
                 cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20) |
-                        ((pop.parent_slot >= 0 ? 1 : 0) << 24);
+                        ((pop.parent_slot >= 0 ? 1 : 0) << 24) | (op_shape4(op, pop) << 25);
                 cw[1] = op.pos;
                 cw[2] = op.tip_load;
                 cw[3] = pop.parent_slot >= 0 ? 1 : 0;

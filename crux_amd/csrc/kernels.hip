@@ -846,91 +846,175 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         }
         float4 acc[CL];
         int32_t ex = 0;
-        auto take = [&](auto Jc, auto &&xc, bool mat) {
+        // d = P_c x_c on the matrix core for child j, the components' chains
+        // interleaved (control flow is uniform across the wave here, as the
+        // matrix instructions need).
+        auto pmul = [&](auto Jc, auto &&xc, crux_f32x4 (&d)[CL]) {
             constexpr int j = decltype(Jc)::value;
-            if (mat) {
-                crux_f32x4 d[CL];
-                static_for<4>([&](auto Kc) {
-                    constexpr int k = decltype(Kc)::value;
-                    static_for<CL>([&](auto Mc) {
-                        constexpr int m = decltype(Mc)::value;
-                        constexpr int q = (j * CL + m) * 4 + k;
-                        const float4 xv = xc(m);
-                        const float xk = k == 0 ? xv.x : k == 1 ? xv.y : k == 2 ? xv.z : xv.w;
-                        d[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(
-                            pr[q >> 3], xk, k == 0 ? crux_f32x4{0.f, 0.f, 0.f, 0.f} : d[m], 3, q & 7, 0);
-                    });
+            static_for<4>([&](auto Kc) {
+                constexpr int k = decltype(Kc)::value;
+                static_for<CL>([&](auto Mc) {
+                    constexpr int m = decltype(Mc)::value;
+                    constexpr int q = (j * CL + m) * 4 + k;
+                    const float4 xv = xc(m);
+                    const float xk = k == 0 ? xv.x : k == 1 ? xv.y : k == 2 ? xv.z : xv.w;
+                    d[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+                        pr[q >> 3], xk, k == 0 ? crux_f32x4{0.f, 0.f, 0.f, 0.f} : d[m], 3, q & 7, 0);
                 });
+            });
+        };
+        auto tip_vec = [&](int j) {
+            const uint32_t mk = tm[j] >> (8 * (lane & 3));
+            return make_float4((float)(mk & 1u), (float)((mk >> 1) & 1u), (float)((mk >> 2) & 1u),
+                               (float)((mk >> 3) & 1u));
+        };
+        // A child cached by an earlier plan (partial re-evaluation): planes
+        // past the slot's C read as zero (buffer range).  The caller waits.
+        auto load_child = [&](int j, float4 (&x)[CL]) -> int32_t {
+            const CRUX_CAS OpChild &ch = op->ch[j];
+            const char *cb = reinterpret_cast<const char *>(ch.clv);
 #pragma unroll
-                for (int m = 0; m < CL; ++m) {
-                    const float4 v = make_float4(d[m].x, d[m].y, d[m].z, d[m].w);
-                    acc[m] = (j == 0) ? v : mul4(acc[m], v);
+            for (int m = 0; m < CL; ++m) {
+                const crux_f32x4 v =
+                    bload4(rsrc_of(cb + (size_t)m * S * 16, (uint32_t)(C - m) * S * 16), vol, 0);
+                x[m] = make_float4(v.x, v.y, v.z, v.w);
+            }
+            return bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
+        };
+        // Two-child updates with a parent and no identity child (most
+        // updates) run a straight-line body specialised for their children's
+        // sources; the control word names the pair.  acc = d0 * d1 and the
+        // exponent sum are formed as in the generic walk below.
+        auto two = [&](auto S0c, auto S1c) {
+            constexpr int s0 = decltype(S0c)::value, s1 = decltype(S1c)::value;
+            float4 xh[2][CL];
+            int32_t eh[2] = {0, 0};
+            if constexpr (s0 == kSrcHbm)
+                eh[0] = load_child(0, xh[0]);
+            if constexpr (s1 == kSrcHbm)
+                eh[1] = load_child(1, xh[1]);
+            if constexpr (s0 == kSrcHbm || s1 == kSrcHbm)
+                __builtin_amdgcn_s_waitcnt(0);
+            crux_f32x4 d[2][CL];
+            static_for<2>([&](auto Jc) {
+                constexpr int j = decltype(Jc)::value;
+                constexpr int sj = j == 0 ? s0 : s1;
+                if constexpr (sj == kSrcTip) {
+                    const float4 t = tip_vec(j);
+                    pmul(Jc, [&](int) { return t; }, d[j]);
+                } else if constexpr (sj == kSrcFwd) {
+                    pmul(Jc, [&](int m) { return yl[m]; }, d[j]);
+                    ex += el;
+                } else if constexpr (sj == kSrcStk) {
+                    pmul(Jc, [&](int m) { return sx[j][m]; }, d[j]);
+                    ex += se[j];
+                } else {
+                    pmul(Jc, [&](int m) { return xh[j][m]; }, d[j]);
+                    ex += eh[j];
                 }
-            } else {
+            });
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                acc[m] = mul4(make_float4(d[0][m].x, d[0][m].y, d[0][m].z, d[0][m].w),
+                              make_float4(d[1][m].x, d[1][m].y, d[1][m].z, d[1][m].w));
+        };
+        using Tip = std::integral_constant<int, kSrcTip>;
+        using Fwd = std::integral_constant<int, kSrcFwd>;
+        using Stk = std::integral_constant<int, kSrcStk>;
+        using Hbm = std::integral_constant<int, kSrcHbm>;
+        auto two_of = [&](auto S0c, int s1) {
+            if (s1 == kSrcTip)
+                two(S0c, Tip{});
+            else if (s1 == kSrcStk)
+                two(S0c, Stk{});
+            else if (s1 == kSrcHbm)
+                two(S0c, Hbm{});
+            else if constexpr (decltype(S0c)::value != kSrcFwd)
+                two(S0c, Fwd{});
+        };
+        const int shape = (w0 >> 25) & 31;
+        if (shape != 0) {
+            const int s0 = (shape - 1) >> 2, s1 = (shape - 1) & 3;
+            if (s0 == kSrcFwd)
+                two_of(Fwd{}, s1);
+            else if (s0 == kSrcTip)
+                two_of(Tip{}, s1);
+            else if (s0 == kSrcStk)
+                two_of(Stk{}, s1);
+            else
+                two_of(Hbm{}, s1);
+        } else {
+            // Generic walk: any child count, identity children, the root.
+            static_for<kOpMaxChild>([&](auto Jc) {
+                constexpr int j = decltype(Jc)::value;
+                if (j >= nch)
+                    return;
+                const int kind = (kinds >> (2 * j)) & 3;
+                float4 x[CL];
+                int32_t e = 0;
+                if (kind == kChildTip) {
+                    const float4 t = tip_vec(j);
+#pragma unroll
+                    for (int m = 0; m < CL; ++m)
+                        x[m] = t;
+                } else if ((fwd >> j) & 1) {
+#pragma unroll
+                    for (int m = 0; m < CL; ++m)
+                        x[m] = yl[m];
+                    e = el;
+                } else if ((fst >> j) & 1) {
+#pragma unroll
+                    for (int m = 0; m < CL; ++m)
+                        x[m] = sx[j][m];
+                    e = se[j];
+                } else {
+                    e = load_child(j, x);
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+                ex += e;
+                float4 v[CL];
+                if (kind != kChildIdent) {
+                    crux_f32x4 d[CL];
+                    pmul(Jc, [&](int m) { return x[m]; }, d);
+#pragma unroll
+                    for (int m = 0; m < CL; ++m)
+                        v[m] = make_float4(d[m].x, d[m].y, d[m].z, d[m].w);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < CL; ++m)
+                        v[m] = x[m];
+                }
 #pragma unroll
                 for (int m = 0; m < CL; ++m)
-                    acc[m] = (j == 0) ? xc(m) : mul4(acc[m], xc(m));
-            }
-        };
-        static_for<kOpMaxChild>([&](auto Jc) {
-            constexpr int j = decltype(Jc)::value;
-            if (j >= nch)
-                return;
-            const int kind = (kinds >> (2 * j)) & 3;
-            const bool mat = kind != kChildIdent;
-            if (kind == kChildTip) {
-                const uint32_t mk = tm[j] >> (8 * (lane & 3));
-                const float4 t = make_float4((float)(mk & 1u), (float)((mk >> 1) & 1u),
-                                             (float)((mk >> 2) & 1u), (float)((mk >> 3) & 1u));
-                take(Jc, [&](int) { return t; }, true);
-            } else if ((fwd >> j) & 1) {
-                take(Jc, [&](int m) { return yl[m]; }, mat);
-                ex += el;
-            } else if ((fst >> j) & 1) {
-                take(Jc, [&](int m) { return sx[j][m]; }, mat);
-                ex += se[j];
-            } else {
-                // Cached by an earlier plan (partial re-evaluation).  Planes
-                // past the slot's C read as zero (buffer range).  The wait
-                // stays inside this branch (see stage_tips).
-                const CRUX_CAS OpChild &ch = op->ch[j];
-                const char *cb = reinterpret_cast<const char *>(ch.clv);
-                float4 x[CL];
-#pragma unroll
-                for (int m = 0; m < CL; ++m) {
-                    const crux_f32x4 v =
-                        bload4(rsrc_of(cb + (size_t)m * S * 16, (uint32_t)(C - m) * S * 16), vol, 0);
-                    x[m] = make_float4(v.x, v.y, v.z, v.w);
-                }
-                ex += bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
-                __builtin_amdgcn_s_waitcnt(0);
-                take(Jc, [&](int m) { return x[m]; }, mat);
-            }
-        });
+                    acc[m] = (j == 0) ? v[m] : mul4(acc[m], v[m]);
+            });
+        }
         // The next update's control words and P image, while this one ends.
         if (oq + 1 < n)
             read_op(buf, oq + 1);
+        // Site maximum over the lane's components, then over the halves, and
+        // the rescaled result (also formed at the root, where nothing uses
+        // it: the loop-carried registers then have one definition).
+        int32_t mb = 0;
+#pragma unroll
+        for (int m = 0; m < CL; ++m)
+            mb = max(max(mb, max(__float_as_int(acc[m].x), __float_as_int(acc[m].y))),
+                     max(__float_as_int(acc[m].z), __float_as_int(acc[m].w)));
+        {
+            uint32_t a0, a1;
+            lane_pair<32>((uint32_t)mb, a0, a1);
+            mb = max((int32_t)a0, (int32_t)a1);
+        }
+        const float mx = __int_as_float(mb);
+        int e2 = 0;
+        if (mx > 0.0f)
+            (void)frexpf(mx, &e2);
+        el = ex + e2;
+#pragma unroll
+        for (int m = 0; m < CL; ++m)
+            yl[m] = make_float4(ldexpf(acc[m].x, -e2), ldexpf(acc[m].y, -e2),
+                                ldexpf(acc[m].z, -e2), ldexpf(acc[m].w, -e2));
         if (has_parent) {
-            // Site maximum over the lane's components, then over the halves.
-            int32_t mb = 0;
-#pragma unroll
-            for (int m = 0; m < CL; ++m)
-                mb = max(max(mb, max(__float_as_int(acc[m].x), __float_as_int(acc[m].y))),
-                         max(__float_as_int(acc[m].z), __float_as_int(acc[m].w)));
-            {
-                uint32_t a0, a1;
-                lane_pair<32>((uint32_t)mb, a0, a1);
-                mb = max((int32_t)a0, (int32_t)a1);
-            }
-            const float mx = __int_as_float(mb);
-            int e2 = 0;
-            if (mx > 0.0f)
-                (void)frexpf(mx, &e2);
-            el = ex + e2;
-#pragma unroll
-            for (int m = 0; m < CL; ++m)
-                yl[m] = make_float4(ldexpf(acc[m].x, -e2), ldexpf(acc[m].y, -e2),
-                                    ldexpf(acc[m].z, -e2), ldexpf(acc[m].w, -e2));
             const int pp = op_pos(pk, kOpMaxChild);
             if (pp >= 0) {
 #pragma unroll
