@@ -6,8 +6,10 @@
 //   CLV slot   : float  clv[S/G][KP/4][64][4]  then  int32 exp[S]   (K != 4;
 //                KP = K rounded up to a multiple of 4, states >= K held at 0;
 //                G = 64/CM sites per tile, lane c*G + s%G as in the kernel)
-//                float  clv[C][S][4]   then   int32 exp[S]   (K == 4: component
-//                planes, so a wave's float4 access covers 1 KiB contiguously)
+//                float  clv[S32/32][CM/2][2][32][4]  then  int32 exp[S]  (K == 4,
+//                CM >= 2: tiles of 32 sites, component h*CM/2 + m of a site in
+//                [m][h]; a wave's float4 store covers 1 KiB contiguously)
+//                float  clv[S][4]  then  int32 exp[S]   (K == 4, one component)
 //   tip rows   : uint8  rows[ntaxa][S64]     K == 4: 4-bit state set; K != 4:
 //                the code index; rows padded to S64 = S rounded up to 64
 //   P image 4  : float  img[p4_regs][64]     K == 4: an update's P matrices as

@@ -308,7 +308,11 @@ static int arena_layout(crux_tree *t, int pol, int C)
     arena_reset(t, pol);
     a.C = C;
     if (t->K == 4) {
-        a.clv_bytes = align_up((size_t)t->S * C * 4 * sizeof(float), 256);
+        // One plane [S][4] for one component; tiles of 32 sites holding all
+        // components of the class otherwise.
+        const int cm = traverse_cmax(4, C);
+        a.clv_bytes = cm <= 1 ? align_up((size_t)t->S * 4 * sizeof(float), 256)
+                              : align_up((size_t)(t->S + 31) / 32 * 32 * cm * 4 * sizeof(float), 256);
     } else {
         // Whole tiles of 64 (site, component) lanes of the component class.
         const int cm = traverse_cmax(t->K, C), g = sites20(cm > 0 ? cm : 1);

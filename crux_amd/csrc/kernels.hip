@@ -705,9 +705,23 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     double sum = 0.0;
     // Byte offsets of the lane's site in the first plane of its half; lanes
     // past the last site store at an offset no buffer range covers.
+    // A slot holds tiles of 32 sites, each [CL][2][32][4] floats: component
+    // h CL + m of site 32 t + i at ((t CL + m) 2 + h) 32 + i, so that one
+    // store (fixed m) of a wave writes 1 KiB and an update 512 CM bytes
+    // contiguously.  Lanes past the last site and components past C (lane
+    // half 1 only) use an offset no buffer range covers: their stores write
+    // nothing and their loads read zero.
     constexpr uint32_t kOut = 0xfffffff0u;
-    const uint32_t vo = act ? (uint32_t)(h * CL * S + s) * 16 : kOut;
-    const uint32_t vol = (uint32_t)(h * CL * S + sc) * 16;
+    uint32_t vo[CL], vol[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+        const bool comp = h == 0 || CL + m < C;
+        const auto at = [&](int site) {
+            return (uint32_t)((((site >> 5) * CL + m) * 2 + h) * 32 + (site & 31)) * 16;
+        };
+        vo[m] = act && comp ? at(s) : kOut;
+        vol[m] = comp ? at(sc) : kOut;
+    }
     const uint32_t voe = (act && h == 0) ? (uint32_t)s * 4 : kOut;
 
     // Both waves of the block order their LDS writes before a barrier; no
@@ -873,10 +887,10 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         auto load_child = [&](int j, float4 (&x)[CL]) -> int32_t {
             const CRUX_CAS OpChild &ch = op->ch[j];
             const char *cb = reinterpret_cast<const char *>(ch.clv);
+            const auto rc = rsrc_of(cb, (uint32_t)E->exp_off);
 #pragma unroll
             for (int m = 0; m < CL; ++m) {
-                const crux_f32x4 v =
-                    bload4(rsrc_of(cb + (size_t)m * S * 16, (uint32_t)(C - m) * S * 16), vol, 0);
+                const crux_f32x4 v = bload4(rc, vol[m], 0);
                 x[m] = make_float4(v.x, v.y, v.z, v.w);
             }
             return bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
@@ -1022,15 +1036,11 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                     stk[wv][pp][m][lane] = yl[m];
                 stke[wv][pp][lane] = el;
             }
-            // Store m of half h writes plane h CL + m: the range of store m
-            // reaches the second half's plane only where that plane exists.
             const char *pp0 = reinterpret_cast<const char *>(par);
+            const auto rp = rsrc_of(pp0, (uint32_t)E->exp_off);
 #pragma unroll
             for (int m = 0; m < CL; ++m)
-                bstore4(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w},
-                        rsrc_of(pp0 + (size_t)m * S * 16,
-                                (CL + m < C ? (uint32_t)(CL + 1) : 1u) * (uint32_t)S * 16),
-                        vo, 0);
+                bstore4(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
             bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), voe, 0);
         } else {
             double L = 0.0;
