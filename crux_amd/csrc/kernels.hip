@@ -215,10 +215,16 @@ __device__ __forceinline__ int32_t bload1(__amdgpu_buffer_rsrc_t r, uint32_t vof
     return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
 }
 
+// Cache policy of a store: 0, or kNT (the non-temporal bit of gfx950's
+// buffer instructions) for CLV stores, which a later evaluation reads back,
+// if at all, long after the caches have turned over.
+constexpr int kNT = 2;
+
+template <int POL = 0>
 __device__ __forceinline__ void bstore4(crux_f32x4 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff, POL);
 }
 
 __device__ __forceinline__ void bstore1(int32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
@@ -615,7 +621,7 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
             const char *pp0 = reinterpret_cast<const char *>(par);
 #pragma unroll
             for (int c = 0; c < CM; ++c)
-                bstore4(crux_f32x4{yl[c].x, yl[c].y, yl[c].z, yl[c].w},
+                bstore4<kNT>(crux_f32x4{yl[c].x, yl[c].y, yl[c].z, yl[c].w},
                         rsrc_of(pp0 + (size_t)c * S * 16, c < C ? (uint32_t)S * 16 : 0u),
                         (uint32_t)s * 16, 0);
             bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), (uint32_t)s * 4, 0);
@@ -1040,7 +1046,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             const auto rp = rsrc_of(pp0, (uint32_t)E->exp_off);
 #pragma unroll
             for (int m = 0; m < CL; ++m)
-                bstore4(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
+                bstore4<kNT>(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
             bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), voe, 0);
         } else {
             double L = 0.0;
@@ -1405,7 +1411,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 const auto rd = rsrc_of(ptr(cw0, 4) + (size_t)tile * NI * 1024);
 #pragma unroll
                 for (int I = 0; I < NI; ++I)
-                    bstore4(yl[I], rd, lane16, I * 1024);
+                    bstore4<kNT>(yl[I], rd, lane16, I * 1024);
                 if (c == 0 && site_ok)
                     bstore1(el, rsrc_of(ptr(cw0, 6)), (uint32_t)t * 4, (uint32_t)tile * G * 4);
             }
