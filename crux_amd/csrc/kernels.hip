@@ -855,15 +855,14 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         float4 sx[kOpMaxChild][CL];
         int32_t se[kOpMaxChild];
         uint32_t tm[kOpMaxChild];
-#pragma unroll
-        for (int j = 0; j < kOpMaxChild; ++j) {
+        auto read_stack = [&](int j) {
             const int p = op_pos(pk, j);
 #pragma unroll
             for (int m = 0; m < CL; ++m)
                 sx[j][m] = stk[wv][p & (D - 1)][m][lane];
             se[j] = stke[wv][p & (D - 1)][lane];
-            tm[j] = tipm[wv][p & (kTipChunk - 1)][(lane & 31) >> 2];
-        }
+        };
+        auto read_tip = [&](int j) { tm[j] = tipm[wv][op_pos(pk, j) & (kTipChunk - 1)][(lane & 31) >> 2]; };
         float4 acc[CL];
         int32_t ex = 0;
         // d = P_c x_c on the matrix core for child j, the components' chains
@@ -907,6 +906,14 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         // exponent sum are formed as in the generic walk below.
         auto two = [&](auto S0c, auto S1c) {
             constexpr int s0 = decltype(S0c)::value, s1 = decltype(S1c)::value;
+            if constexpr (s0 == kSrcStk)
+                read_stack(0);
+            if constexpr (s1 == kSrcStk)
+                read_stack(1);
+            if constexpr (s0 == kSrcTip)
+                read_tip(0);
+            if constexpr (s1 == kSrcTip)
+                read_tip(1);
             float4 xh[2][CL];
             int32_t eh[2] = {0, 0};
             if constexpr (s0 == kSrcHbm)
@@ -965,6 +972,11 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 two_of(Hbm{}, s1);
         } else {
             // Generic walk: any child count, identity children, the root.
+#pragma unroll
+            for (int j = 0; j < kOpMaxChild; ++j) {
+                read_stack(j);
+                read_tip(j);
+            }
             static_for<kOpMaxChild>([&](auto Jc) {
                 constexpr int j = decltype(Jc)::value;
                 if (j >= nch)
