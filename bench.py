@@ -220,7 +220,7 @@ def main():
     # counters read them for this workload (scaled by the copy-kernel
     # calibration), recorded next to the profile summary they come from.
     traffic_hbm = None
-    prof = os.path.join(ROOT, "profiles", "k_trav4_dna_c2.json")
+    prof = os.path.join(ROOT, "profiles", "k_trav4h_dna_c2.json")
     if os.path.exists(prof):
         with open(prof) as f:
             rec = json.load(f)
