@@ -5,11 +5,13 @@
 //              leaf vector of every distinct code instead (tip table).  This
 //              replaces the per-step, per-stripe CxLikPt calls of the
 //              reference (lib/Crux/Tree/CxLik.c:334-347).
-//  k_trav4     The CLV updates of a whole 4-state plan: one wave per 64
-//  k_trav20    sites (k_trav4) or per 64 (site, component) lanes (k_trav20,
-//              20 and 22 states) walks every node update of the plan in
+//  k_trav4h    The CLV updates of a whole 4-state plan: two waves per 64
+//  k_trav4     sites with the components split over lane halves (k_trav4h,
+//  k_trav20    two or more components), one wave per 64 sites (k_trav4, one
+//              component) or per 64 (site, component) lanes (k_trav20, 20
+//              and 22 states) walk every node update of the plan in
 //              post-order, so one launch covers the whole traversal and
-//              every evaluation of a batch with no barrier.  Node update =
+//              every evaluation of a batch.  Node update =
 //              product over children of (P * child) (or a tip lookup), then a
 //              power-of-two rescale of the site so its largest entry is in
 //              [0.5, 1), the exponent added to the children's.  This is the
@@ -306,8 +308,9 @@ __device__ __forceinline__ float4 mul4(float4 a, float4 b)
 // pipeline, and one image register carries 16 columns of P.
 
 // ---------------------------------------------------------------------------
-// 4-state traversal.  A block is one wave: 64 sites, one per lane, with all
-// C mixture components of a site in that lane's registers.  The wave walks
+// 4-state traversal, one component (the lane-halves form below takes two or
+// more).  A block is one wave: 64 sites, one per lane, with all C mixture
+// components of a site in that lane's registers.  The wave walks
 // the whole plan (post-order list of fused node updates) for its sites: one
 // launch covers a full traversal and every evaluation of a batch, and no
 // wave waits for any other -- there is no barrier in the kernel.  Per update
@@ -316,8 +319,9 @@ __device__ __forceinline__ float4 mul4(float4 a, float4 b)
 // multiplies them, takes the site maximum over all components and states in
 // registers, and rescales the site exactly by the power of two that brings it
 // into [0.5, 1), adding the exponent to the children's.  The scaled component
-// planes are stored to the HBM cache slot (layout [C][S][4], 1 KiB
-// contiguous per store) and, when the parent is in the same plan, pushed onto
+// planes are stored to the HBM cache slot (layout [C][S][4], C = 1 where
+// this kernel runs: 1 KiB contiguous per store) and, when the parent is in
+// the same plan, pushed onto
 // the wave's LDS stack; parents pop children from there, so a full traversal
 // reads no CLV back from HBM.  The root update reduces to the
 // pattern-weighted FP64 site lnL (NaN -> -inf, CxLik.c:598-632) and a
