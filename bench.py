@@ -28,6 +28,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -167,7 +179,7 @@ def main():
         log("oracle: lnL %.6f dev %.6f |d| %.3e rel %.3e; %.2f evals/s on %d threads" %
             (ref_l, dev_l, dl_abs, dl_rel, cpu_eps, ncpu))
         cpu = {"value": round(cpu_eps, 3), "unit": "lnL evals/s", "cores": ncpu,
-               "kind": "port",
+               "kind": "port", "cpu_model": _cpu_model(),
                "sample": "%d full-traversal evaluations of the same 128-taxon, %d-pattern "
                          "GTR+G4 workload by the FP64 C restatement of the reference "
                          "engine (oracle/cxlik_oracle.c) with %d stripe threads" % (n, S, ncpu)}
@@ -254,6 +266,7 @@ def main():
                      "traffic": traffic_hbm, "algorithmic_bytes": traffic, "node_updates": nodes,
                      "model_bytes": formula_bytes,
                      "model_GBps": round(formula_bytes / (trav_ms * 1e-3) / 1e9, 1),
+                     "peak_kind": "published MI355X HBM3E bandwidth (spec)",
                      "measured_write_GBps": round(write_peak, 1) if write_peak else None,
                      "frac_of_measured_write": round(achieved / write_peak, 4) if write_peak else None},
         "cpu_baseline": cpu,
