@@ -229,9 +229,10 @@ __device__ __forceinline__ void bstore4(crux_f32x4 v, __amdgpu_buffer_rsrc_t r, 
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff, POL);
 }
 
+template <int POL = 0>
 __device__ __forceinline__ void bstore1(int32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, r, (int)voff, (int)soff, POL);
 }
 
 // The values of a lane and of its partner lane ^ OFF, as an unordered pair
@@ -628,7 +629,7 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
                 bstore4<kNT>(crux_f32x4{yl[c].x, yl[c].y, yl[c].z, yl[c].w},
                         rsrc_of(pp0 + (size_t)c * S * 16, c < C ? (uint32_t)S * 16 : 0u),
                         (uint32_t)s * 16, 0);
-            bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), (uint32_t)s * 4, 0);
+            bstore1<kNT>(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), (uint32_t)s * 4, 0);
         } else {
             double L = 0.0;
 #pragma unroll
@@ -1063,7 +1064,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 #pragma unroll
             for (int m = 0; m < CL; ++m)
                 bstore4<kNT>(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
-            bstore1(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), voe, 0);
+            bstore1<kNT>(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), voe, 0);
         } else {
             double L = 0.0;
 #pragma unroll
@@ -1429,7 +1430,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 for (int I = 0; I < NI; ++I)
                     bstore4<kNT>(yl[I], rd, lane16, I * 1024);
                 if (c == 0 && site_ok)
-                    bstore1(el, rsrc_of(ptr(cw0, 6)), (uint32_t)t * 4, (uint32_t)tile * G * 4);
+                    bstore1<kNT>(el, rsrc_of(ptr(cw0, 6)), (uint32_t)t * 4, (uint32_t)tile * G * 4);
             }
         } else {
             // Weights of the lane's component (all loads issued at once;
