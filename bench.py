@@ -112,10 +112,13 @@ def main():
     log("full: %.1f evals/s, %.3f ms/eval; kernel ms P=%.3f trav=%.3f" %
         (full_eps, 1e3 * dt / args.steps, tm[0], tm[1]))
 
-    # traversal kernel timing over many launches with HIP events
+    # Traversal kernel duration: HIP events around the traversal launch on
+    # the context's stream, averaged over as many evaluations as the timed
+    # region runs (a second pass: the events' own calls stay out of the
+    # evals/s figure).
     ctx.set_timing(True)
     tr = []
-    for _ in range(20):
+    for _ in range(args.steps):
         full()
         tr.append(ctx.last_timing()[1])
     ctx.set_timing(False)
