@@ -143,7 +143,7 @@ struct Entry {
     // Control words of each update, kCtlWords (4-state) or kCtlWords20 per
     // update: nchild | kinds << 8 | from_stack << 16 | forwarded << 20 (bit
     // j: child j is the previous update's result, kept in registers) |
-    // has_parent << 24 | shape << 25 (4 states, see ChildSrc), pos,
+    // has_parent << 24 | shape << 25 (see ChildSrc), pos,
     // tip_load, has_parent, parent (2 words), parent_exp (2 words); for
     // K != 4 then per child j at 8 + 6 j: P image or tip table, CLV, exponents
     // (2 words each; the context's zero block where the child has none).  The
@@ -154,7 +154,7 @@ struct Entry {
     const float *zero;
 };
 
-// Child sources named by the 4-state control word's update shape: shape
+// Child sources named by the control word's update shape: shape
 // 1 + 4 s0 + s1 is a two-child update with a parent whose children come from
 // sources s0 and s1 (no identity child); shape 0 is any other update.
 enum ChildSrc : int {

@@ -529,10 +529,10 @@ struct Planner {
 
 }  // namespace
 
-// Shape of a 4-state update for the traversal's specialised bodies: 1 +
-// 4 s0 + s1 for two children from sources s0, s1 (ChildSrc) under a parent
-// with no identity child, else 0.
-static int op_shape4(const Op &op, const PlanOp &pop)
+// Shape of an update for the traversals' specialised bodies: 1 + 4 s0 + s1
+// for two children from sources s0, s1 (ChildSrc) under a parent with no
+// identity child, else 0.
+static int op_shape(const Op &op, const PlanOp &pop)
 {
     if (op.nchild != 2 || pop.parent_slot < 0)
         return 0;
@@ -1019,7 +1019,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
             auto write_ctl = [&]() {
                 memset(cw, 0, nw * sizeof(int32_t));
                 cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20) |
-                        ((pop.parent_slot >= 0 ? 1 : 0) << 24) | (op_shape4(op, pop) << 25);
+                        ((pop.parent_slot >= 0 ? 1 : 0) << 24) | (op_shape(op, pop) << 25);
                 cw[1] = op.pos;
                 cw[2] = op.tip_load;
                 cw[3] = pop.parent_slot >= 0 ? 1 : 0;
