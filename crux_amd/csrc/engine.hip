@@ -30,9 +30,12 @@
 namespace crux {
 
 int traverse_cmax(int K, int C);
+// Optional events (null: none) are stamped when the kernel starts and ends.
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, int depth, hipStream_t stream);
-void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream);
+                    int blk0, int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
+                    hipEvent_t t1);
+void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream, hipEvent_t t0,
+                 hipEvent_t t1);
 
 static thread_local char g_errbuf[512] = "";
 
@@ -785,8 +788,8 @@ static int complete_slot(crux_ctx *ctx, int k)
     if (S.timed) {
         float a = 0, b = 0, c = 0, d = 0;
         HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&c, ctx->ev[1], ctx->ev[2]));
         HIP_TRY(hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[3]));
         ctx->last_ms[0] = a;
         ctx->last_ms[1] = b;
@@ -1155,25 +1158,37 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     hipStream_t st = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipMemcpyAsync(D, H, plan_bytes, hipMemcpyHostToDevice, st));
-    if (ctx->timing)
+    // Timing: ev[0]/ev[1] stamp the P-matrix kernel's start and end,
+    // ev[2]/ev[3] the first traversal launch's start and the last one's end.
+    const bool tm = ctx->timing;
+    if (ij > 0) {
+        launch_pmat(djobs, (int)ij, shmem, st, tm ? ctx->ev[0] : nullptr, tm ? ctx->ev[1] : nullptr);
+    } else if (tm) {
         HIP_TRY(hipEventRecord(ctx->ev[0], st));
-    if (ij > 0)
-        launch_pmat(djobs, (int)ij, shmem, st);
-    HIP_TRY(hipGetLastError());
-    if (ctx->timing)
         HIP_TRY(hipEventRecord(ctx->ev[1], st));
-    for (Launch &ln : launches) {
+    }
+    HIP_TRY(hipGetLastError());
+    size_t first = launches.size(), last = 0;
+    for (size_t i = 0; i < launches.size(); i++)
+        if (launches[i].nblk > 0) {
+            first = std::min(first, i);
+            last = i;
+        }
+    for (size_t i = 0; i < launches.size(); i++) {
+        const Launch &ln = launches[i];
         if (ln.nblk == 0)
             continue;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
-                            stack_depth(ln.K, ln.CM, ctx->depth4), st) != 0)
+                            stack_depth(ln.K, ln.CM, ctx->depth4), st,
+                            tm && i == first ? ctx->ev[2] : nullptr,
+                            tm && i == last ? ctx->ev[3] : nullptr) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
         HIP_TRY(hipGetLastError());
     }
-    if (ctx->timing)
+    if (tm && first == launches.size()) {
         HIP_TRY(hipEventRecord(ctx->ev[2], st));
-    if (ctx->timing)
         HIP_TRY(hipEventRecord(ctx->ev[3], st));
+    }
     HIP_TRY(hipEventRecord(S.done, st));
     ctx->last_bytes = traffic;
     ctx->last_formula = formula;

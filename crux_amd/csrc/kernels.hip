@@ -25,6 +25,7 @@
 //              host finishes the evaluation's sum once the batch is done.
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <type_traits>
@@ -1483,49 +1484,53 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 // ---------------------------------------------------------------------------
 // Launchers.
 
+// Launches take optional start and stop events (null: none), which the
+// runtime stamps when the kernel itself starts and ends.
+
 template <int CM, int D>
 static void launch_trav4(const Entry *entries, const int32_t *block_entry, int blk0,
-                         int nblocks, hipStream_t stream)
+                         int nblocks, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
     if constexpr (D > stack_depth4_max(CM))
         return;
     else if constexpr (CM == 1)
-        hipLaunchKernelGGL((k_trav4<CM, D>), dim3(nblocks), dim3(kTile4), 0, stream,
-                           entries, block_entry, blk0);
+        hipExtLaunchKernelGGL((k_trav4<CM, D>), dim3(nblocks), dim3(kTile4), 0, stream, t0, t1, 0,
+                              entries, block_entry, blk0);
     else
-        hipLaunchKernelGGL((k_trav4h<CM, D>), dim3(nblocks), dim3(2 * kTile4), 0, stream,
-                           entries, block_entry, blk0);
+        hipExtLaunchKernelGGL((k_trav4h<CM, D>), dim3(nblocks), dim3(2 * kTile4), 0, stream, t0,
+                              t1, 0, entries, block_entry, blk0);
 }
 
 template <int KP, int CM, int D>
 static void launch_trav20(const Entry *entries, const int32_t *block_entry, int blk0,
-                          int nblocks, hipStream_t stream)
+                          int nblocks, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
-    hipLaunchKernelGGL((k_trav20<KP, CM, D>), dim3(nblocks), dim3(64), 0, stream, entries,
-                       block_entry, blk0);
+    hipExtLaunchKernelGGL((k_trav20<KP, CM, D>), dim3(nblocks), dim3(64), 0, stream, t0, t1, 0,
+                          entries, block_entry, blk0);
 }
 
 template <int K, int CM>
 static void launch_traverse(const Entry *entries, const int32_t *block_entry, int blk0,
-                            int nblocks, int depth, hipStream_t stream)
+                            int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
+                            hipEvent_t t1)
 {
     // Stack depth: more slots keep more results in LDS, fewer let more
     // blocks share a CU.
     if constexpr (K == 4) {
         if (depth <= 2)
-            launch_trav4<CM, 2>(entries, block_entry, blk0, nblocks, stream);
+            launch_trav4<CM, 2>(entries, block_entry, blk0, nblocks, stream, t0, t1);
         else if (depth <= 4)
-            launch_trav4<CM, 4>(entries, block_entry, blk0, nblocks, stream);
+            launch_trav4<CM, 4>(entries, block_entry, blk0, nblocks, stream, t0, t1);
         else
-            launch_trav4<CM, 8>(entries, block_entry, blk0, nblocks, stream);
+            launch_trav4<CM, 8>(entries, block_entry, blk0, nblocks, stream, t0, t1);
     } else {
         constexpr int KP = kpad(K);
         if (depth <= 2)
-            launch_trav20<KP, CM, 2>(entries, block_entry, blk0, nblocks, stream);
+            launch_trav20<KP, CM, 2>(entries, block_entry, blk0, nblocks, stream, t0, t1);
         else if (depth <= 4)
-            launch_trav20<KP, CM, 4>(entries, block_entry, blk0, nblocks, stream);
+            launch_trav20<KP, CM, 4>(entries, block_entry, blk0, nblocks, stream, t0, t1);
         else
-            launch_trav20<KP, CM, 8>(entries, block_entry, blk0, nblocks, stream);
+            launch_trav20<KP, CM, 8>(entries, block_entry, blk0, nblocks, stream, t0, t1);
     }
 }
 
@@ -1548,12 +1553,13 @@ int traverse_cmax(int K, int C)
 }
 
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, int depth, hipStream_t stream)
+                    int blk0, int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
+                    hipEvent_t t1)
 {
-#define CRUX_CASE(KK, CC)                                                       \
-    if (K == KK && CM == CC) {                                                  \
-        launch_traverse<KK, CC>(entries, block_entry, blk0, nblocks, depth, stream); \
-        return 0;                                                               \
+#define CRUX_CASE(KK, CC)                                                                 \
+    if (K == KK && CM == CC) {                                                            \
+        launch_traverse<KK, CC>(entries, block_entry, blk0, nblocks, depth, stream, t0, t1); \
+        return 0;                                                                         \
     }
     CRUX_CASE(4, 1) CRUX_CASE(4, 2) CRUX_CASE(4, 4) CRUX_CASE(4, 8) CRUX_CASE(4, 16)
     CRUX_CASE(20, 1) CRUX_CASE(20, 2) CRUX_CASE(20, 4) CRUX_CASE(20, 8)
@@ -1562,9 +1568,10 @@ int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_en
     return -1;
 }
 
-void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream)
+void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream, hipEvent_t t0,
+                 hipEvent_t t1)
 {
-    hipLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), shmem, stream, jobs);
+    hipExtLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), (uint32_t)shmem, stream, t0, t1, 0, jobs);
 }
 
 

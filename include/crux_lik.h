@@ -191,9 +191,11 @@ enum { CRUX_TUNE_STACK_DEPTH = 1 };
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
 
-/* Timing of the last evaluation on the device stream, in milliseconds:
- * [0] P-matrix kernel, [1] traversal kernel(s), [2] after the traversal
- * (nothing: the final sum runs on the host), [3] total.
+/* Timing of the last evaluation on the device stream, in milliseconds, from
+ * events the runtime stamps when each kernel starts and ends: [0] P-matrix
+ * kernel, [1] traversal kernel(s), [2] the gap between the two (launch
+ * latency), [3] total from the P-matrix kernel's start to the traversal's
+ * end (the final sum runs on the host).
  * Only filled when timing was enabled with crux_ctx_set_timing(ctx, 1). */
 int crux_ctx_set_timing(crux_ctx *ctx, int on);
 int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4);
