@@ -683,9 +683,11 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 // broadcast within each half (CBSZ 3: the blocks of lanes 0-31 take block
 // ABID, those of lanes 32-63 block 8 + ABID), so an image register carries 8
 // columns of each half's components; the site maximum pairs the halves with
-// v_permlane32_swap.  Plan walk, forwarding, stack, rescaling, stores and
-// the root reduction follow k_trav4; at the root the halves' component sums
-// are added, and the block's two wave sums are merged in site order.
+// v_permlane32_swap.  Plan walk, forwarding, stack, rescaling and the root
+// reduction follow k_trav4; at the root the halves' component sums are
+// added, and the block's two wave sums are merged in site order.  CLV slots
+// are tiles of 32 sites laid out as the lane halves hold them, so every
+// store writes 1 KiB contiguously; stores are non-temporal.
 template <int CM, int D>
 __global__ void __launch_bounds__(2 * kTile4)
 k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
@@ -719,8 +721,6 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     const int sc = act ? s : S - 1;       // in-bounds site for loads of idle lanes
     const CRUX_CAS Op *ops = cas(E->ops);
     double sum = 0.0;
-    // Byte offsets of the lane's site in the first plane of its half; lanes
-    // past the last site store at an offset no buffer range covers.
     // A slot holds tiles of 32 sites, each [CL][2][32][4] floats: component
     // h CL + m of site 32 t + i at ((t CL + m) 2 + h) 32 + i, so that one
     // store (fixed m) of a wave writes 1 KiB and an update 512 CM bytes
