@@ -34,8 +34,8 @@ int traverse_cmax(int K, int C);
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
                     int blk0, int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
                     hipEvent_t t1);
-void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream, hipEvent_t t0,
-                 hipEvent_t t1);
+void launch_pmat(const PJob *jobs, int njobs, size_t shmem, const void *src, void *dst,
+                 size_t nbytes, hipStream_t stream, hipEvent_t t0, hipEvent_t t1);
 
 static thread_local char g_errbuf[512] = "";
 
@@ -85,7 +85,12 @@ struct crux_ctx {
     // a tree's cache bookkeeping (done when a batch is planned) valid for the
     // next batch before the previous one has finished.
     struct Staging {
-        char *h_plan = nullptr;        // pinned, mirrored by d_plan
+        // The plan: written by the host into coherent pinned memory
+        // (h_plan; the device reads it at dh_plan) and copied to device
+        // memory (d_plan) by the P-matrix launch, which reads its own jobs
+        // straight from h_plan.
+        char *h_plan = nullptr;
+        char *dh_plan = nullptr;
         char *d_plan = nullptr;
         size_t plan_cap = 0;
         float *d_pmat = nullptr;       // P blocks and tip tables
@@ -943,9 +948,11 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
             (void)hipHostFree(S.h_plan);
         dev_free(ctx, S.d_plan, S.plan_cap);
         S.h_plan = nullptr;
+        S.dh_plan = nullptr;
         S.d_plan = nullptr;
         S.plan_cap = 0;
-        HIP_TRY(hipHostMalloc((void **)&S.h_plan, ncap, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&S.h_plan, ncap, hipHostMallocCoherent | hipHostMallocMapped));
+        HIP_TRY(hipHostGetDevicePointer((void **)&S.dh_plan, S.h_plan, 0));
         int rc = dev_alloc(ctx, (void **)&S.d_plan, ncap);
         if (rc)
             return rc;
@@ -973,7 +980,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     int32_t *hblk = (int32_t *)(H + off_blk);
     const Entry *dent = (const Entry *)(D + off_ent);
     const Op *dops = (const Op *)(D + off_ops);
-    const PJob *djobs = (const PJob *)(D + off_jobs);
+    // The P-matrix kernel reads its jobs from the pinned plan itself.
+    const PJob *djobs = (const PJob *)(S.dh_plan + off_jobs);
     const int32_t *dblk = (const int32_t *)(D + off_blk);
 
     size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0, itip = 0, ictl = 0;
@@ -1157,16 +1165,14 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
 
     hipStream_t st = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMemcpyAsync(D, H, plan_bytes, hipMemcpyHostToDevice, st));
-    // Timing: ev[0]/ev[1] stamp the P-matrix kernel's start and end,
-    // ev[2]/ev[3] the first traversal launch's start and the last one's end.
+    // One launch computes the P matrices and copies the plan from the pinned
+    // buffer to device memory (no runtime copy: on this ROCm the launch
+    // after a host-to-device copy costs the host several microseconds).
+    // Timing: ev[0]/ev[1] stamp that kernel's start and end, ev[2]/ev[3]
+    // the first traversal launch's start and the last one's end.
     const bool tm = ctx->timing;
-    if (ij > 0) {
-        launch_pmat(djobs, (int)ij, shmem, st, tm ? ctx->ev[0] : nullptr, tm ? ctx->ev[1] : nullptr);
-    } else if (tm) {
-        HIP_TRY(hipEventRecord(ctx->ev[0], st));
-        HIP_TRY(hipEventRecord(ctx->ev[1], st));
-    }
+    launch_pmat(djobs, (int)ij, shmem, S.dh_plan, D, plan_bytes, st, tm ? ctx->ev[0] : nullptr,
+                tm ? ctx->ev[1] : nullptr);
     HIP_TRY(hipGetLastError());
     size_t first = launches.size(), last = 0;
     for (size_t i = 0; i < launches.size(); i++)

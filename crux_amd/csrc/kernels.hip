@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -112,9 +113,20 @@ __device__ __forceinline__ void nsum_merge(double &s, double &c, double s2, doub
 // ---------------------------------------------------------------------------
 // P matrices and tip tables.
 
-__global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs)
+// Blocks [0, njobs): one P job each.  Blocks past them copy the plan (n16
+// 16-byte words) from pinned host memory to device memory for the
+// traversal launch that follows.
+__global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs, int njobs,
+                                              const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                              int n16)
 {
     extern __shared__ double sm[];
+    if ((int)blockIdx.x >= njobs) {
+        const int nb = (int)gridDim.x - njobs;
+        for (int i = ((int)blockIdx.x - njobs) * 256 + (int)threadIdx.x; i < n16; i += nb * 256)
+            dst[i] = src[i];
+        return;
+    }
     const PJob J = jobs[blockIdx.x];
     const int C = J.C, K = J.K, KK = K * K, st = model_stride(K);
     double *ex = sm;            // [C][K] exp(lambda_k * t_c)
@@ -1568,10 +1580,14 @@ int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_en
     return -1;
 }
 
-void launch_pmat(const PJob *jobs, int njobs, size_t shmem, hipStream_t stream, hipEvent_t t0,
-                 hipEvent_t t1)
+void launch_pmat(const PJob *jobs, int njobs, size_t shmem, const void *src, void *dst,
+                 size_t nbytes, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
-    hipExtLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), (uint32_t)shmem, stream, t0, t1, 0, jobs);
+    // One copy block per 4 KiB of plan, at most 64.
+    const int n16 = (int)(nbytes / 16);
+    const int ncopy = std::max(1, std::min(64, (n16 + 255) / 256));
+    hipExtLaunchKernelGGL(k_pmat, dim3(njobs + ncopy), dim3(256), (uint32_t)shmem, stream, t0, t1, 0,
+                          jobs, njobs, static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
 }
 
 
