@@ -60,9 +60,11 @@ def main(argv=None):
     for _ in range(args.steps):
         full()
     eps = args.steps / (time.perf_counter() - t)
+    # Traversal kernel duration from events the runtime stamps at the
+    # kernel's start and end, averaged over as many evaluations as timed.
     ctx.set_timing(True)
     tr = []
-    for _ in range(5):
+    for _ in range(args.steps):
         full()
         tr.append(ctx.last_timing()[1])
     ctx.set_timing(False)
