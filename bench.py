@@ -150,8 +150,10 @@ def main():
         prob.tree.base = e.ring.node
         return v
 
+    part_nodes = []
     for _ in range(20):
         partial()
+        part_nodes.append(ctx.last_traffic()[2])
     t = time.perf_counter()
     for _ in range(args.partial_steps):
         partial()
@@ -159,7 +161,7 @@ def main():
     log("partial: %.1f evals/s" % part_eps)
 
     # --- FP64 oracle: parity and CPU baseline ---------------------------------
-    cpu = cpu1 = None
+    cpu = cpu1 = cpup = None
     dl_abs = dl_rel = None
     if not args.no_cpu:
         from oracle.oracle import OracleLik
@@ -198,6 +200,18 @@ def main():
         cpu1 = {"value": round(n1 / (time.perf_counter() - t), 3), "unit": "lnL evals/s",
                 "cores": 1, "kind": "port", "sample": "%d evaluations, one thread" % n1}
         log("oracle, one thread: %.2f evals/s" % cpu1["value"])
+        # Partial re-evaluation on the CPU: the same restatement executing as
+        # many CLV updates (plus the root) as the device's partial
+        # re-evaluations above needed on average.
+        k = int(round(float(np.mean(part_nodes))))
+        tp = ora.time_partial(k, reps=20, warmup=3)
+        cpup = {"value": round(1.0 / tp, 2), "unit": "lnL evals/s", "cores": ncpu,
+                "kind": "port",
+                "sample": "median of 20 executions (after 3) of the last %d CLV updates and "
+                          "the root of the same workload's plan on %d stripe threads: the "
+                          "work of a partial re-evaluation (the device's averaged %.1f node "
+                          "updates)" % (k, ncpu, float(np.mean(part_nodes)))}
+        log("oracle, partial: %.1f evals/s" % cpup["value"])
 
     # Write bandwidth the HIP runtime's fill kernel reaches on this GPU
     # (hipMemsetD32Async over 512 MiB, timed with HIP events on the null
@@ -274,6 +288,7 @@ def main():
                      "frac_of_measured_write": round(achieved / write_peak, 4) if write_peak else None},
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
+        "cpu_baseline_partial": cpup,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -409,6 +409,11 @@ class OracleLik:
     def _execute(self, root, stripe_width=None):
         self.prep()
         steps, root_cl, keep = self._plan(root)
+        site, stripe = self._run(steps, root_cl, stripe_width)
+        self._last = (steps, keep)
+        return site, stripe
+
+    def _run(self, steps, root_cl, stripe_width=None):
         n = self.alignment.nchars
         ncomp = len(self.comps)
         if stripe_width is None:
@@ -442,8 +447,35 @@ class OracleLik:
                     ctypes.cast(st, ctypes.c_void_p), _ptr(root_cl[0]),
                     _ptr(root_cl[1]), _ptr(site), _ptr(stripe))
         lib().orc_execute(ctypes.byref(L), self.nthreads)
-        self._last = (steps, keep)
         return site, stripe
+
+    def time_partial(self, nupdates, reps=20, warmup=3, root=None):
+        """Median CPU seconds of a partial re-evaluation's work: the plan's
+        last ``nupdates`` CLV updates and the root's steps, executed on the
+        buffers a full execution has just filled.  The reference's planner
+        recomputes only invalidated CLVs (Lik.pyx:1427-1601), and each of its
+        updates costs the same (patterns x components x states^2) work, so
+        this times the CPU work of a device partial re-evaluation with as
+        many node updates."""
+        import time
+        self.prep()
+        steps, root_cl, keep = self._plan(root)
+        self._run(steps, root_cl)
+        # Steps are grouped by parent CLV in execution order; the root's
+        # group is last.
+        groups = []
+        for st in steps:
+            if not groups or groups[-1][0] is not st[2]:
+                groups.append((st[2], []))
+            groups[-1][1].append(st)
+        tail = [st for _, g in groups[-(nupdates + 1):] for st in g]
+        ts = []
+        for i in range(warmup + reps):
+            t = time.perf_counter()
+            self._run(tail, root_cl)
+            if i >= warmup:
+                ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
 
     def lnL(self, root=None):
         site, stripe = self._execute(root)
