@@ -370,6 +370,9 @@ template <int CM, int D>
 __global__ void __launch_bounds__(kTile4)
 k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
+    // Two or more components run k_trav4h (the loop below is written for any
+    // CM, but only the one-component class is launched).
+    static_assert(CM == 1, "k_trav4 serves the one-component class");
     constexpr int PC = trav4_pchunk<CM>();
     constexpr int NR = p4_regs(CM);                 // P image registers per update
     constexpr int kChunkRows = PC * NR * 16;        // float4s of P per chunk
