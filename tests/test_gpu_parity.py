@@ -261,6 +261,32 @@ def test_batch_and_plain_modes_agree():
         ctx.set_mode(0)
 
 
+def test_large_batch_matches_single_evaluations():
+    # A batch whose plan (operations, control words, P jobs) spans several
+    # hundred KiB: the P-matrix launch copies it to the device over several
+    # passes of its copy blocks.  Every lik's batched lnL equals its own
+    # single evaluation exactly (same kernels, sums in tile order).
+    say("large batch")
+    probs = [synth.make_problem(60 + (i % 5) * 7, 700 + 37 * i, 300 + i) for i in range(40)]
+    liks = []
+    for p in probs:
+        L = Lik(p.tree, p.alignment, ncat=4)
+        synth.configure(L, p.model)
+        liks.append(L)
+    ctx = nat.get_context(0)
+    for L in liks:
+        L._ready(None)
+    got = nat.batch_lnl(ctx, liks, [L.tree.base.id for L in liks])
+    for L, g in zip(liks, got):
+        L.flush()
+        assert L.lnL() == g
+    # And against the FP64 oracle for a few of them.
+    for i in (0, 17, 39):
+        ora = OracleLik(probs[i].fresh_tree(), probs[i].alignment, ncat=4, nthreads=4)
+        synth.configure(ora, probs[i].model)
+        assert close(got[i], ora.lnL())
+
+
 def test_zero_length_conflict_is_neg_inf_or_tiny():
     aln = Alignment(">one\nACGT\n>two\nACGT\n>three\nAAAA\n")
     from crux_amd.tree import Tree
