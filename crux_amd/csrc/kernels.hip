@@ -712,9 +712,9 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     constexpr int HS = kTile4 / 2;                // sites per wave
     // Updates per staged chunk: each chunk ends in a barrier at which both
     // waves wait for their stores to drain, so chunks are as long as the LDS
-    // allows with four blocks per CU (11 at four components with 4 stack
-    // slots: 40 KiB per block).
-    constexpr int PC = CM == 4 && D <= 4 ? 11 : trav4_pchunk<CM>();
+    // allows with four blocks per CU (at most 40 KiB per block: 11 updates
+    // at four components with up to 4 stack slots, 14 at two components).
+    constexpr int PC = CM == 4 && D <= 4 ? 11 : CM == 2 ? 14 : trav4_pchunk<CM>();
     constexpr int NR = p4_regs(CM);               // P image registers per update
     constexpr int kChunkRows = PC * NR * 16;      // float4s of P per chunk
     __shared__ float4 stk[2][D][CL][kTile4];
