@@ -468,9 +468,6 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
     // Control words and P image (one float per lane and register) of the
     // update about to run.
     int4 cwa, cwb;
-    // Up to four components, an update's stack entries are read ahead into
-    // registers; with more, at their use (register budget).
-    constexpr bool kPre = CM <= 4;
     float pr[NR];
     auto read_op = [&](int o) {
         const int rb = (o / PC) & 1, rq = o % PC;
@@ -521,26 +518,19 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
         // whatever the kinds (in bounds whatever the slot holds): they are in
         // flight while the forwarded child is multiplied, and values defined
         // on every path cost no register copies.
-        float4 sx[kOpMaxChild][kPre ? CM : 1];
+        float4 sx[kOpMaxChild][CM];
         int32_t se[kOpMaxChild];
         uint32_t tm[kOpMaxChild];
 #pragma unroll
         for (int j = 0; j < kOpMaxChild; ++j) {
             const int p = op_pos(pk, j);
-            if constexpr (kPre) {
 #pragma unroll
-                for (int c = 0; c < CM; ++c)
-                    sx[j][c] = stk[p & (D - 1)][c][lane];
-            }
+            for (int c = 0; c < CM; ++c)
+                sx[j][c] = stk[p & (D - 1)][c][lane];
             se[j] = stke[p & (D - 1)][lane];
             tm[j] = tipm[p & (kTipChunk - 1)][lane >> 2];
         }
-        auto stack_of = [&](int j, int c) -> float4 {
-            if constexpr (kPre)
-                return sx[j][c];
-            else
-                return stk[op_pos(pk, j) & (D - 1)][c][lane];
-        };
+        auto stack_of = [&](int j, int c) -> float4 { return sx[j][c]; };
         float4 acc[CM];
         int32_t ex = 0;
         // acc = (first child ? v : acc * v), v = P_c x_c (matrix core, the
