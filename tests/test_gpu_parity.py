@@ -261,6 +261,23 @@ def test_batch_and_plain_modes_agree():
         ctx.set_mode(0)
 
 
+@pytest.mark.parametrize("nsites,ncat", [(12, 4), (31, 3), (33, 4), (64, 2), (70, 6), (130, 4)])
+def test_pattern_counts_around_tiles(nsites, ncat):
+    # Pattern counts below, at and just past the 32-site wave and 64-site
+    # tile boundaries (a block's second wave may have no site at all), with
+    # idle components in the last lane half; every other node as root
+    # re-reads cached CLVs from HBM.
+    say("tiles", nsites, ncat)
+    prob = synth.make_problem(9, nsites, 500 + nsites, ncat=ncat, compact=False)
+    dev, ora = _pair(prob, ncat=ncat)
+    b = ora.lnL()
+    assert close(dev.lnL(), b)
+    sd, so = np.array(dev.siteLnLs()), np.array(ora.siteLnLs())
+    assert np.allclose(sd, so, rtol=1e-6, atol=1e-5)
+    for node in list(dev.tree.nodes)[::2]:
+        assert close(dev.lnL(node), b), node
+
+
 def test_large_batch_matches_single_evaluations():
     # A batch whose plan (operations, control words, P jobs) spans several
     # hundred KiB: the P-matrix launch copies it to the device over several
