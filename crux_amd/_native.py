@@ -53,6 +53,7 @@ SIGNATURES = {
     "crux_batch_submit": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _i32p,
                                          ctypes.POINTER(ctypes.c_uint64)]),
     "crux_batch_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, _dp]),
+    "crux_batch_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
     "crux_ctx_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "crux_ctx_get_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
@@ -247,3 +248,11 @@ def batch_wait(ctx, ticket):
     out = np.empty(n)
     check(lib().crux_batch_wait(ctx.handle, t, n, dptr(out)))
     return out
+
+
+def batch_poll(ctx, ticket):
+    """True once a submitted pass has finished (never blocks)."""
+    t, _ = ticket
+    done = ctypes.c_int()
+    check(lib().crux_batch_poll(ctx.handle, t, ctypes.byref(done)))
+    return bool(done.value)

@@ -69,12 +69,11 @@ struct crux_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
-    uint64_t used = 0;
+    uint64_t used = 0;             // bytes allocated (under amu)
     int mode = 0;
     int depth4 = 0;                // traversal LDS stack depth (0 = default)
     float *d_zero = nullptr;       // kZeroBytes of zeros (20/22-state kernel)
     bool timing = false;
-    hipEvent_t ev[4] = {};
     double last_ms[4] = {0, 0, 0, 0};
     uint64_t last_bytes = 0;      // HBM bytes the last evaluation had to move
     uint64_t last_formula = 0;    // same with every child CLV read from HBM
@@ -103,6 +102,9 @@ struct crux_ctx {
         size_t partials_cap = 0;       // doubles
 
         hipEvent_t done = nullptr;
+        // Timing events of this slot's batch: the P-matrix kernel's start
+        // and end, the first traversal's start and the last one's end.
+        hipEvent_t ev[4] = {};
         bool pending = false;
         bool timed = false;
         uint64_t ticket = 0;
@@ -117,20 +119,34 @@ struct crux_ctx {
     int cur = 0;                   // slot the next batch uses
     uint64_t next_ticket = 1;
     std::unordered_map<uint64_t, std::vector<double>> results;  // completed, not collected
-    int ntrees = 0;                // trees alive on this context
+    int ntrees = 0;                // trees alive on this context (under amu)
     bool dying = false;            // destroy requested while trees were alive
     std::mutex mu;                 // one host thread in the context at a time
+    // The allocation counter and tree count: trees of one context may be
+    // created, set up and destroyed on different threads.
+    std::mutex amu;
 };
 
 static int dev_alloc(crux_ctx *ctx, void **p, size_t bytes)
 {
-    if (ctx->budget && ctx->used + bytes > ctx->budget)
-        return set_error(CRUX_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu bytes",
-                         (unsigned long long)ctx->used, (unsigned long long)bytes,
-                         (unsigned long long)ctx->budget);
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMalloc(p, bytes ? bytes : 16));
-    ctx->used += bytes;
+    {
+        std::lock_guard<std::mutex> g(ctx->amu);
+        if (ctx->budget && ctx->used + bytes > ctx->budget)
+            return set_error(CRUX_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu bytes",
+                             (unsigned long long)ctx->used, (unsigned long long)bytes,
+                             (unsigned long long)ctx->budget);
+        ctx->used += bytes;    // reserved before the allocation
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess)
+        e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        std::lock_guard<std::mutex> g(ctx->amu);
+        ctx->used -= bytes;
+        return set_error(e == hipErrorOutOfMemory ? CRUX_ENOMEM : CRUX_EDEVICE,
+                         "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
     return CRUX_OK;
 }
 
@@ -138,6 +154,7 @@ static void dev_free(crux_ctx *ctx, void *p, size_t bytes)
 {
     if (p) {
         (void)hipFree(p);
+        std::lock_guard<std::mutex> g(ctx->amu);
         ctx->used -= bytes;
     }
 }
@@ -175,6 +192,7 @@ constexpr int32_t kRootKey = -1;
 
 struct Arena {
     int C = 0;
+    uint64_t gen = 0;        // bumped whenever the slots are released wholesale
     size_t slot_bytes = 0;   // CLV floats + int32 exponents, each 256-aligned
     size_t clv_bytes = 0;
     int chunk_slots = 0;
@@ -254,6 +272,7 @@ struct crux_lik {
     bool invalidate = true;
     bool root_valid = false;
     int32_t root_tmp_slot = -1;    // scratch CL for roots of degree > kOpMaxChild
+    uint64_t root_tmp_gen = 0;     // arena generation root_tmp_slot belongs to
     double *d_model = nullptr;     // [C][model_stride(K)]
     size_t model_bytes = 0;
     double *d_piw = nullptr;       // [C][K]
@@ -273,6 +292,7 @@ static void arena_reset(crux_tree *t, int pol)
     a.free_slots.clear();
     a.nslots = 0;
     a.C = 0;
+    a.gen++;
     for (RingCache &rc : t->cache[pol]) {
         rc.slot = -1;
         rc.valid = false;
@@ -397,6 +417,7 @@ struct Planner {
                 rc = arena_get(t, pol, &L->root_tmp_slot);
                 if (rc)
                     return;
+                L->root_tmp_gen = t->arena[pol].gen;
             }
             acc_slot = L->root_tmp_slot;
         }
@@ -574,9 +595,12 @@ static int plan_lik(crux_lik *L, int root)
     if (t->arena[L->pol].C != L->C) {
         if (int rc = arena_layout(t, L->pol, L->C))
             return rc;
-        L->root_tmp_slot = -1;
         inval = true;
     }
+    // A scratch root slot from before the arena was last released (by this
+    // lik or another of the polarity) is no longer this lik's.
+    if (L->root_tmp_slot >= 0 && L->root_tmp_gen != t->arena[L->pol].gen)
+        L->root_tmp_slot = -1;
     t->owner[L->pol] = L;
     Planner p{L, t, L->pol, inval, L->ops};
     if (inval)
@@ -774,32 +798,35 @@ static void nsum_merge_host(double &s, double &c, double x, double xc)
         c += xc;
 }
 
-// Finish the batch in staging slot k (and first any older batch in the other
-// slot): wait for its event, hand the results to the likelihoods, keep them
-// under the batch's ticket for crux_batch_wait.
-static int complete_slot(crux_ctx *ctx, int k)
+// Finish the batch of staging set S: wait for its event, hand the results to
+// the likelihoods, keep them under the batch's ticket for crux_batch_wait.
+static int finish(crux_ctx *ctx, Staging &S)
 {
-    Staging &S = ctx->stg[k];
     if (!S.pending)
         return CRUX_OK;
-    Staging &O = ctx->stg[k ^ 1];
-    if (O.pending && O.ticket < S.ticket) {
-        int rc = complete_slot(ctx, k ^ 1);
-        if (rc)
-            return rc;
-    }
+    const hipError_t e = hipEventSynchronize(S.done);
     S.pending = false;
-    HIP_TRY(hipEventSynchronize(S.done));
+    if (e != hipSuccess) {
+        // The batch's results are lost; its likelihoods replan everything.
+        for (auto &it : S.items)
+            it.L->invalidate = true;
+        S.liks.clear();
+        S.items.clear();
+        return set_error(CRUX_EDEVICE, "batch %llu: %s", (unsigned long long)S.ticket,
+                         hipGetErrorString(e));
+    }
     if (S.timed) {
+        // This slot's own events, all complete once `done` is.
         float a = 0, b = 0, c = 0, d = 0;
-        HIP_TRY(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
-        HIP_TRY(hipEventElapsedTime(&c, ctx->ev[1], ctx->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&d, ctx->ev[0], ctx->ev[3]));
-        ctx->last_ms[0] = a;
-        ctx->last_ms[1] = b;
-        ctx->last_ms[2] = c;
-        ctx->last_ms[3] = d;
+        if (hipEventElapsedTime(&a, S.ev[0], S.ev[1]) == hipSuccess &&
+            hipEventElapsedTime(&b, S.ev[2], S.ev[3]) == hipSuccess &&
+            hipEventElapsedTime(&c, S.ev[1], S.ev[2]) == hipSuccess &&
+            hipEventElapsedTime(&d, S.ev[0], S.ev[3]) == hipSuccess) {
+            ctx->last_ms[0] = a;
+            ctx->last_ms[1] = b;
+            ctx->last_ms[2] = c;
+            ctx->last_ms[3] = d;
+        }
     }
     // Each evaluation's lnL: compensated (Neumaier) sum of its waves'
     // partial pairs in tile order, so the result is the same run to run.
@@ -820,6 +847,22 @@ static int complete_slot(crux_ctx *ctx, int k)
     return CRUX_OK;
 }
 
+// Finish the batch in staging slot k, and first any older batch in the other
+// slot (results are handed to the likelihoods in submission order).
+static int complete_slot(crux_ctx *ctx, int k)
+{
+    Staging &S = ctx->stg[k];
+    if (!S.pending)
+        return CRUX_OK;
+    Staging &O = ctx->stg[k ^ 1];
+    if (O.pending && O.ticket < S.ticket) {
+        int rc = complete_slot(ctx, k ^ 1);
+        if (rc)
+            return rc;
+    }
+    return finish(ctx, S);
+}
+
 static int drain(crux_ctx *ctx)
 {
     while (ctx->stg[0].pending || ctx->stg[1].pending) {
@@ -834,17 +877,11 @@ static int drain(crux_ctx *ctx)
     return CRUX_OK;
 }
 
-// Plan and enqueue one batch of planned likelihoods; the results are
-// collected with collect() under the returned ticket.
-static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket)
+// Plan and enqueue one batch of planned likelihoods into staging set S; the
+// results are collected with collect() under the returned ticket.
+static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
+                     uint64_t *ticket)
 {
-    const int slot = ctx->cur;
-    if (ctx->stg[slot].pending) {
-        int rc0 = complete_slot(ctx, slot);
-        if (rc0)
-            return rc0;
-    }
-    Staging &S = ctx->stg[slot];
     // Classes by (K, CM) for the traversal launches.
     struct Item {
         crux_lik *L;
@@ -1171,8 +1208,8 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     // Timing: ev[0]/ev[1] stamp that kernel's start and end, ev[2]/ev[3]
     // the first traversal launch's start and the last one's end.
     const bool tm = ctx->timing;
-    launch_pmat(djobs, (int)ij, shmem, S.dh_plan, D, plan_bytes, st, tm ? ctx->ev[0] : nullptr,
-                tm ? ctx->ev[1] : nullptr);
+    launch_pmat(djobs, (int)ij, shmem, S.dh_plan, D, plan_bytes, st, tm ? S.ev[0] : nullptr,
+                tm ? S.ev[1] : nullptr);
     HIP_TRY(hipGetLastError());
     size_t first = launches.size(), last = 0;
     for (size_t i = 0; i < launches.size(); i++)
@@ -1186,14 +1223,14 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
             continue;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
                             stack_depth(ln.K, ln.CM, ctx->depth4), st,
-                            tm && i == first ? ctx->ev[2] : nullptr,
-                            tm && i == last ? ctx->ev[3] : nullptr) != 0)
+                            tm && i == first ? S.ev[2] : nullptr,
+                            tm && i == last ? S.ev[3] : nullptr) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
         HIP_TRY(hipGetLastError());
     }
     if (tm && first == launches.size()) {
-        HIP_TRY(hipEventRecord(ctx->ev[2], st));
-        HIP_TRY(hipEventRecord(ctx->ev[3], st));
+        HIP_TRY(hipEventRecord(S.ev[2], st));
+        HIP_TRY(hipEventRecord(S.ev[3], st));
     }
     HIP_TRY(hipEventRecord(S.done, st));
     ctx->last_bytes = traffic;
@@ -1206,9 +1243,24 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
     S.items.clear();
     for (size_t ii = 0; ii < items.size(); ii++)
         S.items.push_back({items[ii].L, item_poff[ii], item_ntl[ii]});
-    ctx->cur = slot ^ 1;
     *ticket = S.ticket;
     return CRUX_OK;
+}
+
+// Submit into the next of the two rotating staging slots (the host plans
+// one batch while the device runs the other).
+static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket)
+{
+    const int slot = ctx->cur;
+    if (ctx->stg[slot].pending) {
+        int rc0 = complete_slot(ctx, slot);
+        if (rc0)
+            return rc0;
+    }
+    int rc = submit_to(ctx, ctx->stg[slot], liks, ticket);
+    if (rc == CRUX_OK && ctx->stg[slot].pending)
+        ctx->cur = slot ^ 1;
+    return rc;
 }
 
 // Results of a submitted batch, waiting for it if needed.
@@ -1282,10 +1334,12 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 4 && e == hipSuccess; i++)
-        e = hipEventCreate(&ctx->ev[i]);
-    for (int k = 0; k < 2 && e == hipSuccess; k++)
-        e = hipEventCreateWithFlags(&ctx->stg[k].done, hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        Staging &S = ctx->stg[k];
+        e = hipEventCreateWithFlags(&S.done, hipEventDisableTiming);
+        for (int i = 0; i < 4 && e == hipSuccess; i++)
+            e = hipEventCreate(&S.ev[i]);
+    }
     if (e == hipSuccess)
         e = hipMalloc((void **)&ctx->d_zero, kZeroBytes);
     if (e == hipSuccess)
@@ -1304,11 +1358,14 @@ int crux_ctx_destroy(crux_ctx *ctx)
 {
     if (!ctx)
         return CRUX_OK;
-    if (ctx->ntrees > 0) {
-        // Trees still refer to the context (and their liks to the trees):
-        // the last crux_tree_destroy finishes the job.
-        ctx->dying = true;
-        return CRUX_OK;
+    {
+        std::lock_guard<std::mutex> g(ctx->amu);
+        if (ctx->ntrees > 0) {
+            // Trees still refer to the context (and their liks to the
+            // trees): the last crux_tree_destroy finishes the job.
+            ctx->dying = true;
+            return CRUX_OK;
+        }
     }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
@@ -1321,17 +1378,24 @@ int crux_ctx_destroy(crux_ctx *ctx)
         dev_free(ctx, S.d_pmat, S.pmat_cap * sizeof(float));
         if (S.done)
             (void)hipEventDestroy(S.done);
+        for (int i = 0; i < 4; i++)
+            if (S.ev[i])
+                (void)hipEventDestroy(S.ev[i]);
     }
-    for (int i = 0; i < 4; i++)
-        if (ctx->ev[i])
-            (void)hipEventDestroy(ctx->ev[i]);
+
     (void)hipFree(ctx->d_zero);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CRUX_OK;
 }
 
-uint64_t crux_ctx_bytes(const crux_ctx *ctx) { return ctx ? ctx->used : 0; }
+uint64_t crux_ctx_bytes(const crux_ctx *ctx)
+{
+    if (!ctx)
+        return 0;
+    std::lock_guard<std::mutex> g(const_cast<crux_ctx *>(ctx)->amu);
+    return ctx->used;
+}
 
 int crux_ctx_set_mode(crux_ctx *ctx, int mode)
 {
@@ -1455,7 +1519,10 @@ int crux_tree_create(crux_ctx *ctx, int dim, int nsites, int ntaxa, const uint8_
         crux_tree_destroy(t);
         return set_error(CRUX_EDEVICE, "tree upload: %s", hipGetErrorString(e));
     }
-    ctx->ntrees++;
+    {
+        std::lock_guard<std::mutex> g(ctx->amu);
+        ctx->ntrees++;
+    }
     *out = t;
     return CRUX_OK;
 }
@@ -1479,7 +1546,12 @@ int crux_tree_destroy(crux_tree *t)
         if (t->owner[p])
             t->owner[p]->tree = nullptr;
     delete t;
-    if (--ctx->ntrees == 0 && ctx->dying)
+    bool last;
+    {
+        std::lock_guard<std::mutex> g(ctx->amu);
+        last = --ctx->ntrees == 0 && ctx->dying;
+    }
+    if (last)
         crux_ctx_destroy(ctx);
     return CRUX_OK;
 }
@@ -1664,7 +1736,7 @@ int crux_lik_destroy(crux_lik *L)
         (void)hipStreamSynchronize(ctx->stream);
         if (t->owner[L->pol] == L)
             t->owner[L->pol] = nullptr;
-        if (L->root_tmp_slot >= 0 && t->arena[L->pol].C == L->C)
+        if (L->root_tmp_slot >= 0 && L->root_tmp_gen == t->arena[L->pol].gen)
             arena_put(t, L->pol, L->root_tmp_slot);
         dev_free(ctx, L->d_model, L->model_bytes);
         dev_free(ctx, L->d_piw, sizeof(double) * L->C * t->K);
@@ -1774,6 +1846,7 @@ int crux_lik_flush(crux_lik *L)
     if (!L || !L->tree)
         return set_error(CRUX_EINVAL, "bad arguments");
     crux_tree *t = L->tree;
+    std::lock_guard<std::mutex> g(t->ctx->mu);
     if (t->owner[L->pol] == L) {
         for (size_t r = 0; r < t->cache[L->pol].size(); r++) {
             RingCache &c = t->cache[L->pol][r];
@@ -1832,6 +1905,37 @@ int crux_batch_submit(crux_ctx *ctx, int n, crux_lik *const *liks, const int32_t
         for (crux_lik *L : v)
             L->invalidate = true;
     return rc;
+}
+
+int crux_batch_poll(crux_ctx *ctx, uint64_t ticket, int *done)
+{
+    if (!ctx || !done)
+        return set_error(CRUX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    *done = 0;
+    if (ctx->results.count(ticket)) {
+        *done = 1;
+        return CRUX_OK;
+    }
+    for (int k = 0; k < 2; k++) {
+        Staging &S = ctx->stg[k];
+        if (!S.pending || S.ticket != ticket)
+            continue;
+        const hipError_t e = hipEventQuery(S.done);
+        if (e == hipErrorNotReady)
+            return CRUX_OK;
+        if (e != hipSuccess)
+            return set_error(CRUX_EDEVICE, "batch %llu: %s", (unsigned long long)ticket,
+                             hipGetErrorString(e));
+        // Finished (and so is any older batch): hand the results over now.
+        int rc = complete_slot(ctx, k);
+        if (rc)
+            return rc;
+        *done = 1;
+        return CRUX_OK;
+    }
+    return set_error(CRUX_EINVAL, "unknown or collected batch ticket %llu",
+                     (unsigned long long)ticket);
 }
 
 int crux_batch_wait(crux_ctx *ctx, uint64_t ticket, int n, double *out)

@@ -302,6 +302,14 @@ class Tree:
         elif with_ is not None:
             raise ValueError("Unexpected with_ argument")
 
+    def __getstate__(self):
+        # Edit sinks belong to the device mirrors of this object, not to a
+        # copy.
+        d = self.__dict__.copy()
+        d["_sinks"] = []
+        d["_topo_sinks"] = []
+        return d
+
     # -- ids and serials ---------------------------------------------------
     def _new_node_id(self):
         i = self._next_node_id

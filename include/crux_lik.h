@@ -175,6 +175,12 @@ int crux_batch_submit(crux_ctx *ctx, int n, crux_lik *const *liks,
                       const int32_t *roots, uint64_t *ticket);
 int crux_batch_wait(crux_ctx *ctx, uint64_t ticket, int n, double *out);
 
+/* Non-blocking completion test of a submitted batch: *done = 1 once its
+ * results are ready (crux_batch_wait then returns them at once), 0 while the
+ * device still runs it.  Replaces, for a serving loop, the blocking wait of
+ * CxLikExecute (CxLik.c:914-968) with a poll. */
+int crux_batch_poll(crux_ctx *ctx, uint64_t ticket, int *done);
+
 /* Execution switches of the context (bench and tests):
  *   mode 0: fused (default) -- one traversal launch per batch;
  *   mode 1: plain -- one launch per node update plus one for the root,

@@ -227,6 +227,18 @@ for j in range(4):
                   "evals": [{"root": None, "lnL": float(v), "dec": decimals(v),
                              "sig": sig}]})
 
+# Lik_i: lnL of each tree and of lik.dup() (an independent copy of the tree
+# on the same alignment); trees canonized by the alignment's taxa order (the
+# distance matrix's taxa map in the script).
+src, exp = read("Lik_i.cx"), read("Lik_i.cx.exp")
+fa = fasta_strs(src)[0]
+L = lnl_lines(exp)
+for i, t in enumerate(trees_list(src)):
+    cases.append({"name": "Lik_i[%d]" % i, "fasta": fa, "tree": t, "ops": [],
+                  "evals": [{"root": None, "lnL": L[2 * i][0], "dec": L[2 * i][1]},
+                            {"root": None, "lnL": L[2 * i + 1][0], "dec": L[2 * i + 1][1],
+                             "dup": True}]})
+
 # Rendered trees after deroot() + canonize() (every script prints them with
 # %.7e lengths before evaluating).
 renders = []

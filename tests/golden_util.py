@@ -30,6 +30,17 @@ def run_case(case, make_lik):
     for ev in case["evals"]:
         apply_ops(lik, ev.get("pre", []))
         root = None if ev["root"] is None else nodes[ev["root"]]
+        if ev.get("dup"):
+            # Lik.dup(): an independent tree copy on the same alignment
+            # (Lik.pyx:642-660); objects without it evaluate a fresh
+            # likelihood on a copy of the tree.
+            if hasattr(lik, "dup"):
+                d = lik.dup()
+            else:
+                d = make_lik(tree.dup(), aln, **case.get("ctor", {}))
+                apply_ops(d, case["ops"])
+            yield ev, d.lnL(), None
+            continue
         lnl = lik.lnL(root)
         sites = lik.siteLnLs(root) if "sites" in ev else None
         yield ev, lnl, sites
