@@ -47,6 +47,7 @@ SIGNATURES = {
     "crux_lik_set_model": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
                                           _i32p, _dp, _dp, ctypes.c_int]),
     "crux_lik_lnl": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "crux_lik_lnl_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "crux_lik_site_lnl": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
     "crux_lik_flush": (ctypes.c_int, [_vp]),
     "crux_batch_lnl": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _i32p, _dp]),

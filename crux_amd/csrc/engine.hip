@@ -117,6 +117,13 @@ struct crux_ctx {
         std::vector<Done> items;                       // evaluated in this batch
     } stg[2];
     int cur = 0;                   // slot the next batch uses
+    // crux_lik_lnl_noalloc: a staging set over the device's static scratch
+    // and host memory kept here (no device or pinned allocation per call).
+    Staging fixed_stg;
+    void *scratch_ptr = nullptr;   // this device's copy of `scratch`
+    std::vector<char> fixed_plan;
+    std::vector<double> fixed_part;
+    bool no_alloc = false;         // set while a no-allocation call plans
     uint64_t next_ticket = 1;
     std::unordered_map<uint64_t, std::vector<double>> results;  // completed, not collected
     int ntrees = 0;                // trees alive on this context (under amu)
@@ -129,6 +136,11 @@ struct crux_ctx {
 
 static int dev_alloc(crux_ctx *ctx, void **p, size_t bytes)
 {
+    if (ctx->no_alloc)
+        return set_error(CRUX_ENOMEM,
+                         "a no-allocation evaluation needs %zu more bytes of CLV slots; the "
+                         "tree's CLV pool must already hold every slot the plan uses "
+                         "(evaluate it once with crux_lik_lnl at this root)", bytes);
     {
         std::lock_guard<std::mutex> g(ctx->amu);
         if (ctx->budget && ctx->used + bytes > ctx->budget)
@@ -878,8 +890,12 @@ static int drain(crux_ctx *ctx)
 }
 
 // Plan and enqueue one batch of planned likelihoods into staging set S; the
-// results are collected with collect() under the returned ticket.
-static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
+// results are collected with collect() under the returned ticket.  With
+// `fixed`, S's buffers are the device's static scratch (d_plan, d_pmat,
+// d_partials) and host memory the context keeps (h_plan, h_partials), laid
+// out here per batch; nothing is allocated, and a batch that does not fit
+// the scratch is refused.
+static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik *> &liks,
                      uint64_t *ticket)
 {
     // Classes by (K, CM) for the traversal launches.
@@ -979,7 +995,32 @@ static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
     if (shmem > 64 * 1024)
         return set_error(CRUX_EINVAL, "P-matrix job too large for LDS");
 
-    if (plan_bytes > S.plan_cap) {
+    if (fixed) {
+        // [plan | P matrices | partials] in the static scratch; the plan and
+        // partials have host copies in the context's own host memory.
+        const size_t off_pm = align_up(plan_bytes, 256);
+        const size_t off_part = align_up(off_pm + pm_floats * sizeof(float), 256);
+        const size_t need = off_part + 2 * tiles * sizeof(double);
+        if (need > kScratchBytes)
+            return set_error(CRUX_EINVAL,
+                             "batch needs %zu bytes of scratch; the static scratch holds %zu "
+                             "(use crux_lik_lnl)", need, (size_t)kScratchBytes);
+        char *base = (char *)ctx->scratch_ptr;
+        if (ctx->fixed_plan.size() < plan_bytes)
+            ctx->fixed_plan.resize(plan_bytes);
+        if (ctx->fixed_part.size() < 2 * tiles)
+            ctx->fixed_part.resize(2 * tiles);
+        S.h_plan = ctx->fixed_plan.data();
+        S.dh_plan = nullptr;
+        S.d_plan = base;
+        S.plan_cap = plan_bytes;
+        S.d_pmat = (float *)(base + off_pm);
+        S.pmat_cap = pm_floats;
+        S.h_partials = ctx->fixed_part.data();
+        S.d_partials = (double *)(base + off_part);
+        S.partials_cap = 2 * tiles;
+    }
+    if (!fixed && plan_bytes > S.plan_cap) {
         size_t ncap = std::max(plan_bytes, S.plan_cap * 2);
         if (S.h_plan)
             (void)hipHostFree(S.h_plan);
@@ -995,10 +1036,10 @@ static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
             return rc;
         S.plan_cap = ncap;
     }
-    int rc = grow(ctx, &S.d_pmat, &S.pmat_cap, std::max<size_t>(pm_floats, 64));
+    int rc = fixed ? CRUX_OK : grow(ctx, &S.d_pmat, &S.pmat_cap, std::max<size_t>(pm_floats, 64));
     if (rc)
         return rc;
-    if (2 * tiles > S.partials_cap) {
+    if (!fixed && 2 * tiles > S.partials_cap) {
         const size_t ncap = std::max<size_t>(2 * tiles, 2 * S.partials_cap);
         if (S.h_partials)
             (void)hipHostFree(S.h_partials);
@@ -1017,8 +1058,9 @@ static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
     int32_t *hblk = (int32_t *)(H + off_blk);
     const Entry *dent = (const Entry *)(D + off_ent);
     const Op *dops = (const Op *)(D + off_ops);
-    // The P-matrix kernel reads its jobs from the pinned plan itself.
-    const PJob *djobs = (const PJob *)(S.dh_plan + off_jobs);
+    // The P-matrix kernel reads its jobs from the pinned plan itself (from
+    // the device copy in the scratch when fixed).
+    const PJob *djobs = (const PJob *)((fixed ? D : S.dh_plan) + off_jobs);
     const int32_t *dblk = (const int32_t *)(D + off_blk);
 
     size_t io = 0, ij = 0, ipm = 0, itile = 0, iblk = 0, ient = 0, itip = 0, ictl = 0;
@@ -1208,8 +1250,16 @@ static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
     // Timing: ev[0]/ev[1] stamp that kernel's start and end, ev[2]/ev[3]
     // the first traversal launch's start and the last one's end.
     const bool tm = ctx->timing;
-    launch_pmat(djobs, (int)ij, shmem, S.dh_plan, D, plan_bytes, st, tm ? S.ev[0] : nullptr,
-                tm ? S.ev[1] : nullptr);
+    if (fixed) {
+        // A stream-ordered copy into the scratch: the previous call's
+        // kernels have read their plan by the time it runs.
+        HIP_TRY(hipMemcpyAsync(D, H, plan_bytes, hipMemcpyHostToDevice, st));
+        launch_pmat(djobs, (int)ij, shmem, nullptr, nullptr, 0, st, tm ? S.ev[0] : nullptr,
+                    tm ? S.ev[1] : nullptr);
+    } else {
+        launch_pmat(djobs, (int)ij, shmem, S.dh_plan, D, plan_bytes, st, tm ? S.ev[0] : nullptr,
+                    tm ? S.ev[1] : nullptr);
+    }
     HIP_TRY(hipGetLastError());
     size_t first = launches.size(), last = 0;
     for (size_t i = 0; i < launches.size(); i++)
@@ -1232,6 +1282,9 @@ static int submit_to(crux_ctx *ctx, Staging &S, std::vector<crux_lik *> &liks,
         HIP_TRY(hipEventRecord(S.ev[2], st));
         HIP_TRY(hipEventRecord(S.ev[3], st));
     }
+    if (fixed)
+        HIP_TRY(hipMemcpyAsync(S.h_partials, S.d_partials, 2 * tiles * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(S.done, st));
     ctx->last_bytes = traffic;
     ctx->last_formula = formula;
@@ -1257,7 +1310,7 @@ static int submit(crux_ctx *ctx, std::vector<crux_lik *> &liks, uint64_t *ticket
         if (rc0)
             return rc0;
     }
-    int rc = submit_to(ctx, ctx->stg[slot], liks, ticket);
+    int rc = submit_to(ctx, ctx->stg[slot], false, liks, ticket);
     if (rc == CRUX_OK && ctx->stg[slot].pending)
         ctx->cur = slot ^ 1;
     return rc;
@@ -1299,6 +1352,37 @@ static int execute(crux_ctx *ctx, std::vector<crux_lik *> &liks, double *out)
 }
 
 // ---------------------------------------------------------------------------
+// Static scratch of crux_lik_lnl_noalloc: the batch's plan, P matrices and
+// per-wave partial sums, kScratchBytes on every device the library runs on,
+// held until the process ends.  Host code uses the address
+// hipGetSymbolAddress returns on each device, never the array's name.
+
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];
+// Calls on one device share its scratch: they are serialized here.
+static std::mutex g_scratch_use[kMaxDevices];
+
+// With the context's device current: this device's address of `scratch`.
+static int scratch_setup(crux_ctx *ctx)
+{
+    if (ctx->device >= kMaxDevices)
+        return set_error(CRUX_EINVAL, "device %d: no scratch", ctx->device);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(CRUX_EDEVICE, "hipGetSymbolAddress(scratch): %s",
+                             hipGetErrorString(e));
+        g_scratch_dev[ctx->device] = p;
+    }
+    ctx->scratch_ptr = g_scratch_dev[ctx->device];
+    return CRUX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI.
 
 extern "C" {
@@ -1334,8 +1418,8 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int k = 0; k < 2 && e == hipSuccess; k++) {
-        Staging &S = ctx->stg[k];
+    for (int k = 0; k < 3 && e == hipSuccess; k++) {
+        Staging &S = k < 2 ? ctx->stg[k] : ctx->fixed_stg;
         e = hipEventCreateWithFlags(&S.done, hipEventDisableTiming);
         for (int i = 0; i < 4 && e == hipSuccess; i++)
             e = hipEventCreate(&S.ev[i]);
@@ -1349,6 +1433,11 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
             (void)hipFree(ctx->d_zero);
         delete ctx;
         return set_error(CRUX_EDEVICE, "context setup: %s", hipGetErrorString(e));
+    }
+    if (int rc = scratch_setup(ctx)) {
+        (void)hipFree(ctx->d_zero);
+        delete ctx;
+        return rc;
     }
     *out = ctx;
     return CRUX_OK;
@@ -1382,7 +1471,11 @@ int crux_ctx_destroy(crux_ctx *ctx)
             if (S.ev[i])
                 (void)hipEventDestroy(S.ev[i]);
     }
-
+    if (ctx->fixed_stg.done)
+        (void)hipEventDestroy(ctx->fixed_stg.done);
+    for (int i = 0; i < 4; i++)
+        if (ctx->fixed_stg.ev[i])
+            (void)hipEventDestroy(ctx->fixed_stg.ev[i]);
     (void)hipFree(ctx->d_zero);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1936,6 +2029,45 @@ int crux_batch_poll(crux_ctx *ctx, uint64_t ticket, int *done)
     }
     return set_error(CRUX_EINVAL, "unknown or collected batch ticket %llu",
                      (unsigned long long)ticket);
+}
+
+int crux_lik_lnl_noalloc(crux_lik *L, int root, double *lnl)
+{
+    if (!L || !L->tree || !lnl)
+        return set_error(CRUX_EINVAL, "bad arguments");
+    crux_tree *t = L->tree;
+    crux_ctx *ctx = t->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!ctx->scratch_ptr)
+        return set_error(CRUX_ESTATE, "no scratch on device %d", ctx->device);
+    if (t->arena[L->pol].C != L->C)
+        return set_error(CRUX_ENOMEM,
+                         "the tree's CLV pool is laid out for %d components, the model has %d; "
+                         "re-laying it out frees and allocates (evaluate with crux_lik_lnl)",
+                         t->arena[L->pol].C, L->C);
+    std::lock_guard<std::mutex> gs(g_scratch_use[ctx->device]);
+    // Batches in flight hand their results over first (they may hold this
+    // likelihood, whose value must come out in call order).
+    int rc = drain(ctx);
+    if (rc)
+        return rc;
+    ctx->no_alloc = true;
+    rc = plan_lik(L, root);
+    ctx->no_alloc = false;
+    if (rc)
+        return rc;
+    std::vector<crux_lik *> v{L};
+    uint64_t ticket = 0;
+    rc = submit_to(ctx, ctx->fixed_stg, true, v, &ticket);
+    if (rc == CRUX_OK)
+        rc = finish(ctx, ctx->fixed_stg);
+    if (rc == CRUX_OK)
+        rc = collect(ctx, ticket, lnl, 1);
+    if (rc) {
+        L->invalidate = true;
+        return rc;
+    }
+    return CRUX_OK;
 }
 
 int crux_batch_wait(crux_ctx *ctx, uint64_t ticket, int n, double *out)

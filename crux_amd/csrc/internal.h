@@ -8,6 +8,10 @@
 namespace crux {
 
 constexpr int kMaxStates = 64;   // largest state count the host math accepts
+constexpr int kMaxDevices = 64;  // devices the static scratch is set up for
+// Static scratch of crux_lik_lnl_noalloc per device: plan, P matrices and
+// per-wave partial sums of one evaluation.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
 constexpr int kMaxChild = 3;     // children fused into one node update
 
 // Record a formatted detail message for crux_last_error() and return status.

@@ -145,6 +145,21 @@ int crux_lik_set_model(crux_lik *lik, int nmodels, const double *evec,
  * with compensated FP64 accumulation.  Synchronous. */
 int crux_lik_lnl(crux_lik *lik, int root_node, double *lnl);
 
+/* No counterpart in the reference: crux_lik_lnl without any device or
+ * pinned-memory allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * evaluation's plan, P matrices and per-wave partial sums go into a static
+ * scratch of 16 MiB per device, held by the library until the process ends;
+ * the CLV slots are the tree's own.  Fails with CRUX_EINVAL when the
+ * evaluation needs more scratch than that, and with CRUX_ENOMEM when the
+ * tree's CLV pool lacks a slot the plan needs or is laid out for another
+ * component count (an evaluation with crux_lik_lnl at that root sets it up).
+ * Every call on a device shares its scratch, which keeps the last call's
+ * contents (nothing in it needs to start at zero); the library serializes
+ * such calls within a process, since overlapping ones would race.
+ * Synchronous. */
+int crux_lik_lnl_noalloc(crux_lik *lik, int root_node, double *lnl);
+
 /* Pattern-weighted site lnLs of the last evaluation at root_node
  * (Lik.siteLnLs, Lik.pyx:1739-1763); evaluates first when dirty. */
 int crux_lik_site_lnl(crux_lik *lik, int root_node, double *out, int n);
