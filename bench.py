@@ -1,16 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the tree-likelihood hot path on one MI355X.
 
-Workload (default): 128-taxon Yule tree, 100,000 simulated DNA sites
-compressed to ~60k patterns with integer weights, GTR+Gamma4 (alpha 0.5),
-FP32 CLVs with power-of-two site scaling, compensated FP64 site sums.  A
-step is one full-traversal lnL evaluation (every CLV recomputed) with all
-data resident in HBM.  Also reported: single-branch partial re-evaluation
-throughput (re-root at a changed edge, as an MC3 branch-length move does),
-the plain one-launch-per-node path, |dlnL| against the FP64 oracle, the
-roofline of the traversal kernel and the oracle's CPU throughput.
+Headline workload (``value``): the single large DNA likelihood -- 128-taxon
+Yule tree, 100,000 simulated sites compressed to ~60k patterns with integer
+weights, GTR+Gamma4 (alpha 0.5), FP32 CLVs with power-of-two site scaling,
+compensated FP64 site sums.  A step is one full-traversal lnL evaluation
+(every CLV recomputed) with all data resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Also reported in the same JSON line:
+  * single-branch partial re-evaluation (re-root at a changed edge, as an
+    MC3 branch-length move does) and the plain one-launch-per-node path;
+  * the traversal kernel's roofline (algorithmic bytes / kernel time);
+  * amino acids, 256 taxa x 20,000 sites, 20 states +G4: evals/s, HBM and
+    matrix-core fractions, |dlnL| against the FP64 oracle;
+  * batched MC3, 32 runs x 4 heated replicas on 64-512 taxa x 1k-50k sites:
+    fused batched passes (host/device overlapped and one pass per step),
+    one fused call per evaluation, and one launch per node update;
+  * served likelihood: Poisson arrivals of 70% single-branch / 30% full-tree
+    requests on 16 resident trees of 32-1,024 taxa x 500-200k sites, DNA and
+    amino acids: throughput and p50/p99 latency per arrival rate;
+  * the FP64 CPU baseline (the oracle's C restatement of the reference's
+    engine) on the host cores of the same machine: median of 20 evaluations
+    after 3 warm-ups, at one thread and at every available core.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--skip c3,c4,c5,cpu]
 """
 
 import argparse
@@ -26,9 +39,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_MATRIX_PEAK_TFLOPS = 157.3
 
 
-def _cpu_model():
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
     """The host CPU's model name (first processor in /proc/cpuinfo)."""
     try:
         with open("/proc/cpuinfo") as f:
@@ -40,41 +58,24 @@ def _cpu_model():
     return None
 
 
-def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+def available_cores():
+    """Cores this process may use: its CPU affinity, capped by a cgroup CPU
+    quota when one is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(math.ceil(int(quota) / int(period)))))
+    except (OSError, ValueError):
+        pass
+    return n
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--partial-steps", type=int, default=400)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stack-depth", type=int, default=None,
-                    help="4-state kernel LDS stack depth (2, 4, 8; default: tuned)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run device-to-device copies of MIB MiB (the reference "
-                         "kernel for scaling HBM byte counters) and exit")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the likelihood on a single "
-              "device (see DESIGN.md for how the path shards); --gpus %d is not "
-              "supported" % args.gpus, file=sys.stderr)
-        return 2
-    if args.copy_calibration:
-        import torch
-        n = args.copy_calibration << 20
-        a = torch.empty(n, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_each_way": n, "copies": 5}))
-        return 0
+# ---------------------------------------------------------------------------
+# Single large DNA likelihood (the headline).
 
+def bench_dna(args, out):
     from crux_amd import _native as nat
     from crux_amd import synth
     from crux_amd.lik import Lik
@@ -82,8 +83,7 @@ def main():
     t0 = time.time()
     prob = synth.make_problem(128, 100000, 2, mean_len=0.025)
     S = prob.alignment.nchars
-    log("problem: 128 taxa, %d patterns, %.1fs" % (S, time.time() - t0))
-
+    log("dna: 128 taxa, %d patterns, %.1fs" % (S, time.time() - t0))
     ctx = nat.get_context(0)
     if args.stack_depth is not None:
         ctx.set_tuning(nat.CRUX_TUNE_STACK_DEPTH, args.stack_depth)
@@ -92,7 +92,6 @@ def main():
     root = prob.tree.base
     lnl0 = lik.lnL(root)
 
-    # --- full traversals (fused) --------------------------------------------
     def full():
         lik.flush()
         return lik.lnL(root)
@@ -109,35 +108,41 @@ def main():
         full()
     dt = time.perf_counter() - t
     full_eps = args.steps / dt
-    log("full: %.1f evals/s, %.3f ms/eval; kernel ms P=%.3f trav=%.3f" %
-        (full_eps, 1e3 * dt / args.steps, tm[0], tm[1]))
+    log("dna full: %.1f evals/s, %.3f ms/eval" % (full_eps, 1e3 * dt / args.steps))
 
-    # Traversal kernel duration: HIP events around the traversal launch on
-    # the context's stream, averaged over as many evaluations as the timed
-    # region runs (a second pass: the events' own calls stay out of the
-    # evals/s figure).
+    # Traversal kernel duration: HIP events the runtime stamps at the
+    # kernel's start and end on the context's stream, averaged over as many
+    # evaluations as the timed region ran (a second pass, so the events stay
+    # out of the evals/s figure).
     ctx.set_timing(True)
-    tr = []
-    for _ in range(args.steps):
+    tr, pm, gap = [], [], []
+    for _ in range(max(args.steps, 20)):
         full()
-        tr.append(ctx.last_timing()[1])
+        ms = ctx.last_timing()
+        pm.append(ms[0])
+        tr.append(ms[1])
+        gap.append(ms[2])
     ctx.set_timing(False)
     trav_ms = float(np.mean(tr))
     achieved = traffic / (trav_ms * 1e-3) / 1e9
+    log("dna traversal %.4f ms, P-matrix %.4f ms, gap %.4f ms: %.1f GB/s" %
+        (trav_ms, float(np.mean(pm)), float(np.mean(gap)), achieved))
 
-    # --- plain path: one launch per node update ---------------------------
+    # Plain path: one launch per node update.
     ctx.set_mode(1)
-    for _ in range(3):
-        full()
-    t = time.perf_counter()
-    n_plain = max(10, args.steps // 4)
-    for _ in range(n_plain):
-        full()
-    plain_eps = n_plain / (time.perf_counter() - t)
-    ctx.set_mode(0)
-    log("plain: %.1f evals/s" % plain_eps)
+    try:
+        for _ in range(3):
+            full()
+        n_plain = max(10, args.steps // 2)
+        t = time.perf_counter()
+        for _ in range(n_plain):
+            full()
+        plain_eps = n_plain / (time.perf_counter() - t)
+    finally:
+        ctx.set_mode(0)
+    log("dna plain: %.1f evals/s" % plain_eps)
 
-    # --- single-branch partial re-evaluation -----------------------------
+    # Single-branch partial re-evaluation, re-rooted at the changed edge.
     rng = np.random.default_rng(7)
     edges = list(prob.tree.edges)
     lam = 2.0 * math.log(1.6)
@@ -158,92 +163,7 @@ def main():
     for _ in range(args.partial_steps):
         partial()
     part_eps = args.partial_steps / (time.perf_counter() - t)
-    log("partial: %.1f evals/s" % part_eps)
-
-    # --- FP64 oracle: parity and CPU baseline ---------------------------------
-    cpu = cpu1 = cpup = None
-    dl_abs = dl_rel = None
-    if not args.no_cpu:
-        from oracle.oracle import OracleLik
-        ncpu = min(os.cpu_count() or 1, 16)
-        prob2 = synth.make_problem(128, 100000, 2, mean_len=0.025)
-        ora = OracleLik(prob2.tree, prob2.alignment, ncat=4, nthreads=ncpu)
-        synth.configure(ora, prob2.model)
-        lik2 = Lik(prob2.fresh_tree(), prob2.alignment, ncat=4)
-        synth.configure(lik2, prob2.model)
-        dev_l = lik2.lnL()
-        t = time.perf_counter()
-        ref_l = ora.lnL()
-        n = 1
-        while time.perf_counter() - t < args.cpu_seconds and n < 50:
-            ora.lnL()
-            n += 1
-        cpu_eps = n / (time.perf_counter() - t)
-        dl_abs = abs(dev_l - ref_l)
-        dl_rel = dl_abs / abs(ref_l)
-        log("oracle: lnL %.6f dev %.6f |d| %.3e rel %.3e; %.2f evals/s on %d threads" %
-            (ref_l, dev_l, dl_abs, dl_rel, cpu_eps, ncpu))
-        cpu = {"value": round(cpu_eps, 3), "unit": "lnL evals/s", "cores": ncpu,
-               "kind": "port", "cpu_model": _cpu_model(),
-               "sample": "%d full-traversal evaluations of the same 128-taxon, %d-pattern "
-                         "GTR+G4 workload by the FP64 C restatement of the reference "
-                         "engine (oracle/cxlik_oracle.c) with %d stripe threads" % (n, S, ncpu)}
-        # The same restatement on one thread (a few evaluations).
-        ora1 = OracleLik(prob2.fresh_tree(), prob2.alignment, ncat=4, nthreads=1)
-        synth.configure(ora1, prob2.model)
-        t = time.perf_counter()
-        ora1.lnL()
-        n1 = 1
-        while time.perf_counter() - t < args.cpu_seconds / 4 and n1 < 5:
-            ora1.lnL()
-            n1 += 1
-        cpu1 = {"value": round(n1 / (time.perf_counter() - t), 3), "unit": "lnL evals/s",
-                "cores": 1, "kind": "port", "sample": "%d evaluations, one thread" % n1}
-        log("oracle, one thread: %.2f evals/s" % cpu1["value"])
-        # Partial re-evaluation on the CPU: the same restatement executing as
-        # many CLV updates (plus the root) as the device's partial
-        # re-evaluations above needed on average.
-        k = int(round(float(np.mean(part_nodes))))
-        tp = ora.time_partial(k, reps=20, warmup=3)
-        cpup = {"value": round(1.0 / tp, 2), "unit": "lnL evals/s", "cores": ncpu,
-                "kind": "port",
-                "sample": "median of 20 executions (after 3) of the last %d CLV updates and "
-                          "the root of the same workload's plan on %d stripe threads: the "
-                          "work of a partial re-evaluation (the device's averaged %.1f node "
-                          "updates)" % (k, ncpu, float(np.mean(part_nodes)))}
-        log("oracle, partial: %.1f evals/s" % cpup["value"])
-
-    # Write bandwidth the HIP runtime's fill kernel reaches on this GPU
-    # (hipMemsetD32Async over 512 MiB, timed with HIP events on the null
-    # stream): the practical ceiling of an almost write-only kernel, next to
-    # the published peak.
-    write_peak = None
-    try:
-        import ctypes
-        hip = ctypes.CDLL("libamdhip64.so")
-        nbytes = 512 << 20
-        buf, e0, e1 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
-        ok = hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) == 0
-        ok = ok and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
-        if ok:
-            fill = lambda: hip.hipMemsetD32Async(buf, ctypes.c_int(0x3F800000),
-                                                 ctypes.c_size_t(nbytes // 4), None)
-            for _ in range(3):
-                fill()
-            hip.hipEventRecord(e0, None)
-            for _ in range(10):
-                fill()
-            hip.hipEventRecord(e1, None)
-            hip.hipEventSynchronize(e1)
-            ms = ctypes.c_float()
-            hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-            write_peak = nbytes * 10 / (ms.value * 1e-3) / 1e9
-            hip.hipEventDestroy(e0)
-            hip.hipEventDestroy(e1)
-            hip.hipFree(buf)
-            log("fill kernel: %.1f GB/s" % write_peak)
-    except OSError as exc:
-        log("fill kernel not measured: %s" % exc)
+    log("dna partial: %.1f evals/s (%.1f node updates)" % (part_eps, float(np.mean(part_nodes))))
 
     # HBM bytes per traversal launch as rocprofv3's FETCH_SIZE + WRITE_SIZE
     # counters read them for this workload (scaled by the copy-kernel
@@ -256,40 +176,350 @@ def main():
         if rec["workload"] == {"taxa": 128, "patterns": S, "categories": 4}:
             traffic_hbm = int(rec["traffic_bytes"])
 
-    line = {
-        "metric": "lnL evals/s (full traversal, 128 taxa x ~60k patterns GTR+G4)",
+    out.update({
         "value": round(full_eps, 2),
-        "unit": "lnL evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(1e3 / full_eps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 CLV / f64 P,log-sum",
-        "data": "synthetic (Yule tree, GTR+G4 simulated sites, seeded)",
         "config": {"workload": "single large DNA likelihood: 128 taxa, 100,000 sites -> "
                                "%d patterns, GTR+G4 alpha=0.5, full traversal" % S,
                    "patterns": S, "taxa": 128, "categories": 4},
         "partial_evals_per_s": round(part_eps, 2),
+        "partial_node_updates": round(float(np.mean(part_nodes)), 2),
         "plain_gpu_evals_per_s": round(plain_eps, 2),
+        "speedup_vs_plain_gpu": round(full_eps / plain_eps, 2),
         "lnL": lnl0,
-        "dlnL_abs": dl_abs,
-        "dlnL_rel": dl_rel,
-        "kernel_ms": {"pmat": round(float(tm[0]), 4), "traverse": round(trav_ms, 4)},
+        "kernel_ms": {"pmat": round(float(np.mean(pm)), 4), "traverse": round(trav_ms, 4),
+                      "launch_gap": round(float(np.mean(gap)), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_hbm, "algorithmic_bytes": traffic, "node_updates": nodes,
                      "model_bytes": formula_bytes,
                      "model_GBps": round(formula_bytes / (trav_ms * 1e-3) / 1e9, 1),
-                     "peak_kind": "published MI355X HBM3E bandwidth (spec)",
-                     "measured_write_GBps": round(write_peak, 1) if write_peak else None,
-                     "frac_of_measured_write": round(achieved / write_peak, 4) if write_peak else None},
-        "cpu_baseline": cpu,
-        "cpu_baseline_1thread": cpu1,
-        "cpu_baseline_partial": cpup,
+                     "peak_kind": "published MI355X HBM3E bandwidth (spec)"},
+    })
+    return prob, int(round(float(np.mean(part_nodes))))
+
+
+def measured_write_peak(out):
+    """Write bandwidth of the HIP runtime's fill kernel on this GPU
+    (hipMemsetD32Async over 512 MiB, HIP events on the null stream): the
+    practical ceiling of an almost write-only kernel."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError as exc:
+        log("fill kernel not measured: %s" % exc)
+        return
+    nbytes = 512 << 20
+    buf, e0, e1 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    ok = hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    if not ok:
+        return
+    fill = lambda: hip.hipMemsetD32Async(buf, ctypes.c_int(0x3F800000),
+                                         ctypes.c_size_t(nbytes // 4), None)
+    for _ in range(3):
+        fill()
+    hip.hipEventRecord(e0, None)
+    for _ in range(10):
+        fill()
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    peak = nbytes * 10 / (ms.value * 1e-3) / 1e9
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(buf)
+    log("fill kernel: %.1f GB/s" % peak)
+    rf = out["roofline"]
+    rf["measured_write_GBps"] = round(peak, 1)
+    rf["frac_of_measured_write"] = round(rf["achieved"] / peak, 4)
+
+
+# ---------------------------------------------------------------------------
+# FP64 CPU baseline and parity.
+
+def cpu_baseline(args, out, prob2, part_k, aa_prob):
+    """The oracle's FP64 C restatement of CxLikExecute on this host, median
+    of 20 after 3 warm-ups (CL buffers reused as the reference reuses its
+    CL objects), at one thread and at every available core; also the FP64
+    lnL each device result is compared with."""
+    from crux_amd import synth
+    from oracle.oracle import OracleLik
+
+    ncpu = available_cores()
+    model = cpu_model()
+
+    def oracle(prob, nthreads):
+        o = OracleLik(prob.fresh_tree(), prob.alignment, ncat=4, nthreads=nthreads)
+        synth.configure(o, prob.model)
+        return o
+
+    res = {}
+    # Headline workload, all cores and one thread.
+    o = oracle(prob2, ncpu)
+    ref = o.lnL()
+    tn = o.time_full(reps=20, warmup=3)
+    o1 = oracle(prob2, 1)
+    t1 = o1.time_full(reps=20, warmup=3)
+    tp = o.time_partial(part_k, reps=20, warmup=3)
+    log("cpu dna: %.3f s (%d threads), %.3f s (1 thread), partial %.4f s" % (tn, ncpu, t1, tp))
+    res["dna_ref_lnL"] = ref
+    out["cpu_baseline"] = {
+        "value": round(1.0 / tn, 3), "unit": "lnL evals/s", "cores": ncpu, "kind": "port",
+        "cpu_model": model,
+        "sample": "median of 20 full-traversal evaluations (after 3 warm-ups) of the same "
+                  "128-taxon, %d-pattern GTR+G4 workload by the FP64 C restatement of the "
+                  "reference engine (oracle/cxlik_oracle.c) with %d stripe threads"
+                  % (prob2.alignment.nchars, ncpu)}
+    out["cpu_baseline_1thread"] = {
+        "value": round(1.0 / t1, 3), "unit": "lnL evals/s", "cores": 1, "kind": "port",
+        "sample": "median of 20 full evaluations after 3 warm-ups, one thread"}
+    out["cpu_baseline_partial"] = {
+        "value": round(1.0 / tp, 2), "unit": "lnL evals/s", "cores": ncpu, "kind": "port",
+        "sample": "median of 20 executions (after 3) of the last %d CLV updates and the root "
+                  "of the workload's plan: the work of a partial re-evaluation (the device's "
+                  "mean node-update count)" % part_k}
+    # The reference test size (16 taxa, 1,000 sites).
+    p1 = synth.make_problem(16, 1000, 1)
+    c1n = oracle(p1, ncpu).time_full(reps=20, warmup=3)
+    c1s = oracle(p1, 1).time_full(reps=20, warmup=3)
+    out["cpu_baseline_c1"] = {
+        "workload": "16 taxa, 1,000 sites -> %d patterns, GTR+G4" % p1.alignment.nchars,
+        "evals_per_s_all_cores": round(1.0 / c1n, 1), "evals_per_s_1thread": round(1.0 / c1s, 1),
+        "cores": ncpu, "lnL": oracle(p1, 1).lnL()}
+    # Amino acids.
+    if aa_prob is not None:
+        oa = oracle(aa_prob, ncpu)
+        res["aa_ref_lnL"] = oa.lnL()
+        ta = oa.time_full(reps=20, warmup=3)
+        ta1 = oracle(aa_prob, 1).time_full(reps=3, warmup=1)
+        res["aa_cpu"] = {"value": round(1.0 / ta, 3), "unit": "lnL evals/s", "cores": ncpu,
+                         "kind": "port", "value_1thread": round(1.0 / ta1, 3),
+                         "sample": "median of 20 after 3 warm-ups (%d threads); one thread: "
+                                   "median of 3 after 1" % ncpu}
+        log("cpu aa: %.3f s (%d threads), %.3f s (1 thread)" % (ta, ncpu, ta1))
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Amino acids, 20 states.
+
+def bench_aa(args):
+    from crux_amd import _native as nat
+    from crux_amd import synth
+    from crux_amd.lik import Lik
+
+    t0 = time.time()
+    prob = synth.make_problem(256, 20000, 4, kind="aa")
+    S = prob.alignment.nchars
+    log("aa: 256 taxa, %d patterns, %.1fs" % (S, time.time() - t0))
+    ctx = nat.get_context(0)
+    lik = Lik(prob.tree, prob.alignment, ncat=4)
+    synth.configure(lik, prob.model)
+    root = prob.tree.base
+
+    def full():
+        lik.flush()
+        return lik.lnL(root)
+
+    lnl = full()
+    for _ in range(3):
+        full()
+    n = max(10, args.steps)
+    t = time.perf_counter()
+    for _ in range(n):
+        full()
+    eps = n / (time.perf_counter() - t)
+    ctx.set_timing(True)
+    tr = []
+    for _ in range(n):
+        full()
+        tr.append(ctx.last_timing()[1])
+    ctx.set_timing(False)
+    trav = float(np.mean(tr))
+    nbytes, _, nodes = ctx.last_traffic()
+    # Matrix-core FP32 work: per internal child edge and component, a 20x20
+    # matrix times a 20-vector per pattern (tip children are table rows).
+    flops = 2.0 * 20 * 20 * S * 4 * (prob.alignment.ntaxa - 3)
+    tflops = flops / (trav * 1e-3) / 1e12
+    gbs = nbytes / (trav * 1e-3) / 1e9
+    log("aa full: %.1f evals/s; traversal %.4f ms; %.1f GB/s; %.1f TFLOP/s" % (eps, trav, gbs, tflops))
+    return prob, {
+        "workload": "amino acids: 256 taxa, 20,000 sites -> %d patterns, LogNormal "
+                    "exchangeabilities, Dirichlet(1) freqs, +G4 alpha=0.5, full traversal" % S,
+        "evals_per_s": round(eps, 2), "traverse_ms": round(trav, 4), "node_updates": nodes,
+        "lnL": lnl,
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes},
+        "matrix_core": {"tflops": round(tflops, 2), "peak_tflops_fp32": FP32_MATRIX_PEAK_TFLOPS,
+                        "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4)},
     }
+
+
+# ---------------------------------------------------------------------------
+# Batched MC3.
+
+def bench_mc3(args):
+    from crux_amd import mc3
+    from crux_amd import synth
+    from oracle.oracle import OracleLik
+
+    t0 = time.time()
+    probs = mc3.make_problems(32)
+    log("mc3: 32 problems, %.1fs" % (time.time() - t0))
+    mc = mc3.Mc3Batch(probs, ncoupled=4)
+    mc.initialize()
+    mc.step(batched=True)
+
+    def rate(fn, steps):
+        e0 = mc.evals
+        t = time.perf_counter()
+        fn(steps)
+        return (mc.evals - e0) / (time.perf_counter() - t)
+
+    def stepped(k, batched=True):
+        for _ in range(k):
+            mc.step(batched=batched)
+
+    one_pass = rate(stepped, args.mc3_steps)
+    overlapped = rate(mc.run, args.mc3_steps)
+    fused_single = rate(lambda k: stepped(k, batched=False), 2)
+    mc.ctx.set_mode(1)
+    try:
+        plain = rate(lambda k: stepped(k, batched=False), 2)
+    finally:
+        mc.ctx.set_mode(0)
+    # Tracked lnL of a sample of chains against the FP64 oracle on their
+    # current trees.
+    worst = 0.0
+    for c in mc.chains[::37]:
+        o = OracleLik(c.tree.dup(), c.lik.alignment, ncat=4, nthreads=available_cores())
+        synth.configure(o, probs[c.run].model)
+        ref = o.lnL()
+        worst = max(worst, abs(c.lnL - ref) / abs(ref))
+    log("mc3: overlapped %.0f, one pass per step %.0f, fused one call per eval %.0f, "
+        "plain %.0f evals/s; worst |dlnL|/|lnL| %.2e" % (overlapped, one_pass, fused_single,
+                                                          plain, worst))
+    return {"workload": "batched MC3: 32 runs x 4 heated replicas (128 chains), 64-512 taxa x "
+                        "1k-50k sites log-uniform, GTR+G4, branch-length + NNI moves",
+            "batched_evals_per_s": round(overlapped, 1),
+            "batched_one_pass_per_step_evals_per_s": round(one_pass, 1),
+            "fused_one_call_per_eval_evals_per_s": round(fused_single, 1),
+            "plain_evals_per_s": round(plain, 1),
+            "speedup_vs_plain": round(overlapped / plain, 2),
+            "speedup_vs_fused_unbatched": round(overlapped / fused_single, 2),
+            "dlnL_rel_max_sample": worst}
+
+
+# ---------------------------------------------------------------------------
+# Served likelihood.
+
+def bench_serve(args):
+    from crux_amd import _native as nat
+    from crux_amd import serve
+
+    t0 = time.time()
+    sessions = serve.make_sessions(16)
+    log("serve: 16 sessions, %.1fs" % (time.time() - t0))
+    ctx = nat.get_context(0)
+    srv = serve.Server(sessions, ctx)
+    srv.run(serve.make_requests(sessions, 64, 1e9, seed=1))
+    sweep = []
+    for r in args.serve_rates:
+        n = int(min(args.serve_requests, max(200, 2.0 * r)))
+        reqs = serve.make_requests(sessions, n, r, seed=int(r) % 1000 + 7)
+        b0, q0 = srv.batches, srv.batched_requests
+        t = time.perf_counter()
+        srv.run(reqs)
+        s = serve.summarize(reqs, time.perf_counter() - t)
+        s["offered_rps"] = r
+        s["mean_batch"] = round((srv.batched_requests - q0) / max(1, srv.batches - b0), 1)
+        log("serve rate %g: %s" % (r, s))
+        sweep.append(s)
+    return {"workload": "served likelihood: Poisson arrivals, 70% single-branch / 30% full, "
+                        "16 resident trees of 32-1,024 taxa x 500-200k sites, DNA and 20-state "
+                        "amino acids 1:1, GTR+G4",
+            "sweep": sweep}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--partial-steps", type=int, default=400)
+    ap.add_argument("--mc3-steps", type=int, default=10)
+    ap.add_argument("--serve-rates", default="500,2000,8000,100000")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--skip", default="", help="comma list of c3,c4,c5,cpu to leave out")
+    ap.add_argument("--stack-depth", type=int, default=None,
+                    help="4-state kernel LDS stack depth (2, 4, 8; default: tuned)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (the reference "
+                         "kernel for scaling HBM byte counters) and exit")
+    args = ap.parse_args()
+    args.serve_rates = [float(x) for x in args.serve_rates.split(",") if x]
+    skip = set(x for x in args.skip.split(",") if x)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the likelihood on a single "
+              "device (see DESIGN.md for how the path shards); --gpus %d is not "
+              "supported" % args.gpus, file=sys.stderr)
+        return 2
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n, "copies": 5}))
+        return 0
+
+    line = {
+        "metric": "lnL evals/s (full traversal, 128 taxa x ~60k patterns GTR+G4)",
+        "value": None,
+        "unit": "lnL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 CLV / f64 P,log-sum",
+        "data": "synthetic (Yule trees, GTR+G4 / 20-state sites simulated, seeded)",
+    }
+    prob, part_k = bench_dna(args, line)
+    measured_write_peak(line)
+    aa_prob, aa = (None, None) if "c4" in skip else bench_aa(args)
+    mc3r = None if "c3" in skip else bench_mc3(args)
+    srv = None if "c5" in skip else bench_serve(args)
+    refs = {}
+    if "cpu" not in skip:
+        from crux_amd import synth
+        prob2 = synth.make_problem(128, 100000, 2, mean_len=0.025)
+        refs = cpu_baseline(args, line, prob2, part_k, aa_prob)
+        ref = refs["dna_ref_lnL"]
+        line["dlnL_abs"] = abs(line["lnL"] - ref)
+        line["dlnL_rel"] = line["dlnL_abs"] / abs(ref)
+        line["speedup_vs_cpu"] = round(line["value"] / line["cpu_baseline"]["value"], 1)
+        line["partial_speedup_vs_cpu"] = round(
+            line["partial_evals_per_s"] / line["cpu_baseline_partial"]["value"], 1)
+        log("dna |dlnL| %.3e rel %.3e" % (line["dlnL_abs"], line["dlnL_rel"]))
+    if aa is not None:
+        if "aa_ref_lnL" in refs:
+            aa["dlnL_abs"] = abs(aa["lnL"] - refs["aa_ref_lnL"])
+            aa["dlnL_rel"] = aa["dlnL_abs"] / abs(refs["aa_ref_lnL"])
+            aa["cpu_baseline"] = refs["aa_cpu"]
+            aa["speedup_vs_cpu"] = round(aa["evals_per_s"] / refs["aa_cpu"]["value"], 1)
+        line["amino_acids"] = aa
+    if mc3r is not None:
+        line["mc3"] = mc3r
+    if srv is not None:
+        line["serve"] = srv
     print(json.dumps(line), flush=True)
     return 0
 

@@ -98,7 +98,15 @@ def yule_tree(ntaxa, rng, mean_len=0.1, newick=False):
 
 
 def simulate(rooted, model, nsites, rng, taxa=None):
-    """Alignment simulated down a rooted tree under ``model``."""
+    """Alignment simulated down a rooted tree under ``model``.
+
+    Each site draws a Gamma category; along an edge the child state of a
+    site is the first state whose cumulative P(t) row (of the site's
+    category and parent state) reaches a uniform draw.  The inversion starts
+    from a guide table of 4K cells per row (the first state whose cumulative
+    probability reaches the cell's lower end) and steps forward from there,
+    about one step per site, so an edge costs O(sites) rather than
+    O(sites x states)."""
     V, Vi, w, pin, qn = nat.q_decomp(model.rtri, model.pi)
     if model.ncat > 1 and not math.isinf(model.alpha):
         rates = nat.gamma_rates(model.alpha, model.ncat, False)
@@ -107,6 +115,10 @@ def simulate(rooted, model, nsites, rng, taxa=None):
     cat = rng.integers(len(rates), size=nsites)
     states = np.frombuffer("".join(model.charType.get().pcodes()).encode(), dtype=np.uint8)
     K = len(model.pi)
+    C = len(rates)
+    M = 4 * K
+    grid = np.arange(M) / M
+    rowbase = cat * K
     seqs = {}
     base = rooted.base
     root_seq = rng.choice(K, size=nsites, p=pin)
@@ -115,16 +127,23 @@ def simulate(rooted, model, nsites, rng, taxa=None):
         seqs[base.taxon] = root_seq
     while stack:
         ring, pseq = stack.pop()
-        seq = np.empty(nsites, dtype=np.int64)
-        u = rng.random(nsites)
+        cdf = np.empty((C * K, K))
         for ci, r in enumerate(rates):
-            sel = np.nonzero(cat == ci)[0]
-            if len(sel) == 0:
-                continue
             P = nat.pt(V, Vi, w, ring.edge.length * r * qn)
-            cdf = np.cumsum(P, axis=1)
-            cdf[:, -1] = 1.0
-            seq[sel] = (u[sel, None] > cdf[pseq[sel]]).sum(axis=1)
+            c = np.maximum.accumulate(np.cumsum(P, axis=1), axis=1)
+            c[:, -1] = 1.0
+            cdf[ci * K:(ci + 1) * K] = c
+        guide = (cdf[:, None, :] < grid[None, :, None]).sum(axis=2)
+        flat = cdf.ravel()
+        row = rowbase + pseq
+        u = rng.random(nsites)
+        seq = guide[row, (u * M).astype(np.int64)]
+        base_k = row * K
+        # Step forward only the sites that are not there yet.
+        act = np.nonzero(flat[base_k + seq] < u)[0]
+        while act.size:
+            seq[act] += 1
+            act = act[flat[base_k[act] + seq[act]] < u[act]]
         if ring.node.taxon is not None:
             seqs[ring.node.taxon] = seq
         stack.extend((r.other, seq) for r in ring.siblings())

@@ -185,6 +185,11 @@ class OracleLik:
         self.wNorm = 0.0
         self.nthreads = max(1, int(nthreads))
         self._rows = None
+        # CL buffers per ring (and the root's), kept between evaluations as
+        # the reference keeps its CL objects on Ring.aux (Lik.pyx:32-108);
+        # like its malloc'd cLMat they are not cleared: every entry an
+        # evaluation reads it has written first.
+        self._cls = {}
         for _ in range(nmodels):
             self.addModel(1.0, ncat, catMedian, invar)
 
@@ -339,8 +344,11 @@ class OracleLik:
         steps = []
         keep = []
 
-        def new_cl():
-            cl = (np.zeros((n, dn)), np.zeros(n))
+        def new_cl(key):
+            cl = self._cls.get(key)
+            if cl is None or cl[0].shape != (n, dn):
+                cl = (np.empty((n, dn)), np.empty(n))
+                self._cls[key] = cl
             keep.append(cl)
             return cl
 
@@ -365,7 +373,7 @@ class OracleLik:
             if deg == 1:
                 cl_of(ring)
                 return
-            cl = new_cl()
+            cl = new_cl(ring.id)
             ring_cls[ring.id] = cl
             for r in ring.siblings():
                 recurse(r.other)
@@ -381,7 +389,7 @@ class OracleLik:
 
         if root is None:
             root = self.tree.base
-        root_cl = new_cl()
+        root_cl = new_cl("root")
         ring = root.ring
         deg = root.degree
         if deg == 1:
@@ -448,6 +456,18 @@ class OracleLik:
                     _ptr(root_cl[1]), _ptr(site), _ptr(stripe))
         lib().orc_execute(ctypes.byref(L), self.nthreads)
         return site, stripe
+
+    def time_full(self, reps=20, warmup=3, root=None):
+        """Median CPU seconds of a full evaluation (plan and execution, CL
+        buffers reused), after ``warmup`` untimed ones."""
+        import time
+        ts = []
+        for i in range(warmup + reps):
+            t = time.perf_counter()
+            self.lnL(root)
+            if i >= warmup:
+                ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
 
     def time_partial(self, nupdates, reps=20, warmup=3, root=None):
         """Median CPU seconds of a partial re-evaluation's work: the plan's
