@@ -345,6 +345,27 @@ def bench_aa(args):
     tflops = flops / (trav * 1e-3) / 1e12
     gbs = nbytes / (trav * 1e-3) / 1e9
     log("aa full: %.1f evals/s; traversal %.4f ms; %.1f GB/s; %.1f TFLOP/s" % (eps, trav, gbs, tflops))
+    # Opt-in 16-bit inputs to the matrix core (FP32 accumulation; CLVs
+    # stay FP32): traversal time and lnL of each, compared with the FP64
+    # oracle below.
+    precs = {}
+    for name, prec in (("bf16", nat.CRUX_PREC_BF16), ("f16", nat.CRUX_PREC_F16),
+                       ("f16x3", nat.CRUX_PREC_F16X3)):
+        ctx.set_tuning(nat.CRUX_TUNE_PRECISION, prec)
+        v = full()
+        for _ in range(3):
+            full()
+        ctx.set_timing(True)
+        tp = []
+        for _ in range(n):
+            full()
+            tp.append(ctx.last_timing()[1])
+        ctx.set_timing(False)
+        ms = float(np.mean(tp))
+        precs[name] = {"lnL": v, "traverse_ms": round(ms, 4),
+                       "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log("aa %s: traversal %.4f ms, lnL %.6f" % (name, ms, v))
+    ctx.set_tuning(nat.CRUX_TUNE_PRECISION, nat.CRUX_PREC_F32)
     return prob, {
         "workload": "amino acids: 256 taxa, 20,000 sites -> %d patterns, LogNormal "
                     "exchangeabilities, Dirichlet(1) freqs, +G4 alpha=0.5, full traversal" % S,
@@ -354,6 +375,7 @@ def bench_aa(args):
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes},
         "matrix_core": {"tflops": round(tflops, 2), "peak_tflops_fp32": FP32_MATRIX_PEAK_TFLOPS,
                         "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4)},
+        "input_precisions": precs,
     }
 
 
@@ -515,6 +537,8 @@ def main():
             aa["dlnL_rel"] = aa["dlnL_abs"] / abs(refs["aa_ref_lnL"])
             aa["cpu_baseline"] = refs["aa_cpu"]
             aa["speedup_vs_cpu"] = round(aa["evals_per_s"] / refs["aa_cpu"]["value"], 1)
+            for v in aa["input_precisions"].values():
+                v["dlnL_rel"] = abs(v["lnL"] - refs["aa_ref_lnL"]) / abs(refs["aa_ref_lnL"])
         line["amino_acids"] = aa
     if mc3r is not None:
         line["mc3"] = mc3r

@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(_HERE, "libcrux_amd.so")
 CRUX_OK, CRUX_EINVAL, CRUX_ENOMEM, CRUX_ENEGLEN, CRUX_EDECOMP, CRUX_EDEVICE, \
     CRUX_ETAXON, CRUX_ESTATE = range(8)
 CRUX_TUNE_STACK_DEPTH = 1
+CRUX_TUNE_PRECISION = 2
+CRUX_PREC_F32, CRUX_PREC_BF16, CRUX_PREC_F16, CRUX_PREC_F16X3 = range(4)
 
 #: Every entry point the header declares, with (restype, argtypes).
 _vp = ctypes.c_void_p

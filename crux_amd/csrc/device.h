@@ -91,6 +91,33 @@ __host__ __device__ constexpr int img_floats(int KP, int CM) { return ((img_nv(K
 
 __host__ __device__ inline int tile_sites(int K, int CM) { return K == 4 ? kTile4 : sites20(CM); }
 
+// Input precision of the 20/22-state P * x products (the context's
+// CRUX_TUNE_PRECISION).  FP32 inputs run on v_mfma_f32_4x4x1f32, one column
+// per instruction; 16-bit inputs on v_mfma_f32_4x4x4 (bf16 or f16), four
+// columns per instruction, with FP32 accumulation: bfloat16, float16, or
+// float16 split into high and low parts (x = xh + xl and P = Ph + Pl with
+// xl = f16(x - xh), P x ~ Ph xh + Ph xl + Pl xh).  CLVs are stored in FP32
+// whatever the precision.
+enum Prec20 : int32_t { kPrecF32 = 0, kPrecBF16 = 1, kPrecF16 = 2, kPrecF16x3 = 3 };
+
+// 16-bit P images: the A operand of v_mfma_f32_4x4x4 is four values of a row
+// (four consecutive columns) per lane, two per dword.  Dword v = (I * groups
+// + g) * 2 + p of lane c * (64/CM) + 4 a + i holds P_c[4 I + i][(g * cols +
+// a) * 4 + 2 p + e] in its half e (low half first), cols = 16/CM blocks per
+// component group; the split image is the high image, padded to whole
+// 16-byte lane quads, followed by the low.
+__host__ __device__ constexpr int img16_groups(int KP, int CM) { return (KP + 4 * img_cols(CM) - 1) / (4 * img_cols(CM)); }
+__host__ __device__ constexpr int img16_nv(int KP, int CM) { return (KP / 4) * img16_groups(KP, CM) * 2; }
+// Dwords of one 16-bit image padded to whole 16-byte lane quads (the low
+// image of the split precision starts there).
+__host__ __device__ constexpr int img16_nvp(int KP, int CM) { return (img16_nv(KP, CM) + 3) & ~3; }
+// 32-bit words of one P image at a precision.
+__host__ __device__ constexpr int img_words(int KP, int CM, int prec)
+{
+    return prec == kPrecF32 ? img_floats(KP, CM)
+                            : (prec == kPrecF16x3 ? 2 : 1) * img16_nvp(KP, CM) * 64;
+}
+
 // Largest per-lane LDS stack depth of the 4-state traversal for a component
 // class CM (keeps the stack at or below 32 KiB per block).
 __host__ __device__ constexpr int stack_depth4_max(int CM) { return CM <= 4 ? 8 : (CM == 8 ? 4 : 2); }
@@ -209,6 +236,7 @@ struct PJob {
     int32_t cm;            // component class
     int32_t kp;            // padded state count (K != 4)
     int32_t child;         // the child's index in its update (4-state image)
+    int32_t prec;          // K != 4: Prec20 of the image
 };
 
 __host__ __device__ inline int model_stride(int K) { return 2 * K * K + K + 1; }

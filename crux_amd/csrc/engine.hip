@@ -32,7 +32,7 @@ namespace crux {
 int traverse_cmax(int K, int C);
 // Optional events (null: none) are stamped when the kernel starts and ends.
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
+                    int blk0, int nblocks, int depth, int prec, hipStream_t stream, hipEvent_t t0,
                     hipEvent_t t1);
 void launch_pmat(const PJob *jobs, int njobs, size_t shmem, const void *src, void *dst,
                  size_t nbytes, hipStream_t stream, hipEvent_t t0, hipEvent_t t1);
@@ -72,6 +72,7 @@ struct crux_ctx {
     uint64_t used = 0;             // bytes allocated (under amu)
     int mode = 0;
     int depth4 = 0;                // traversal LDS stack depth (0 = default)
+    int prec = kPrecF32;           // input precision of 20/22-state products
     float *d_zero = nullptr;       // kZeroBytes of zeros (20/22-state kernel)
     bool timing = false;
     double last_ms[4] = {0, 0, 0, 0};
@@ -133,6 +134,16 @@ struct crux_ctx {
     // created, set up and destroyed on different threads.
     std::mutex amu;
 };
+
+// LDS stack depth of the traversal kernel a batch runs with: the tuned
+// depth, or 4 for the 16-bit-input 20/22-state kernels (instantiated at that
+// depth only).  The planner assigns stack slots for the same depth.
+static int trav_depth(const crux_ctx *ctx, int K, int CM)
+{
+    if (K != 4 && ctx->prec != kPrecF32)
+        return 4;
+    return stack_depth(K, CM, ctx->depth4);
+}
 
 static int dev_alloc(crux_ctx *ctx, void **p, size_t bytes)
 {
@@ -283,6 +294,7 @@ struct crux_lik {
     bool have_model = false;
     bool invalidate = true;
     bool root_valid = false;
+    int prec = kPrecF32;           // precision the cached CLVs were computed at (K != 4)
     int32_t root_tmp_slot = -1;    // scratch CL for roots of degree > kOpMaxChild
     uint64_t root_tmp_gen = 0;     // arena generation root_tmp_slot belongs to
     double *d_model = nullptr;     // [C][model_stride(K)]
@@ -608,6 +620,11 @@ static int plan_lik(crux_lik *L, int root)
         if (int rc = arena_layout(t, L->pol, L->C))
             return rc;
         inval = true;
+    }
+    // CLVs cached at another input precision are stale.
+    if (t->K != 4 && L->prec != t->ctx->prec) {
+        inval = true;
+        L->prec = t->ctx->prec;
     }
     // A scratch root slot from before the arena was last released (by this
     // lik or another of the polarity) is no longer this lik's.
@@ -951,7 +968,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         // stack; one-launch-per-update mode cannot (LDS does not outlive a
         // launch).
         if (!plain)
-            schedule_stack(L->ops, stack_depth(K, it.CM, ctx->depth4));
+            schedule_stack(L->ops, trav_depth(ctx, K, it.CM));
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
         nops += L->ops.size();
@@ -969,7 +986,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                     shmem = std::max(shmem, (size_t)C * K * sizeof(double));
                 } else {
                     pm_floats += align_up(c.kind == kChildTip ? (size_t)t->ncodes * it.CM * kpad(K)
-                                                              : (size_t)img_floats(kpad(K), it.CM),
+                                                              : (size_t)img_words(kpad(K), it.CM, ctx->prec),
                                           64);
                     shmem = std::max(shmem, ((size_t)C * K + (size_t)C * K * K) * sizeof(double));
                 }
@@ -1170,12 +1187,13 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                     jb.cm = CM;
                     jb.kp = KP;
                     jb.child = (int32_t)j;
+                    jb.prec = K == 4 ? kPrecF32 : ctx->prec;
                     if (K == 4) {
                         jb.out = S.d_pmat + pdense + olocal * p4_floats(CM);
                     } else {
                         jb.out = S.d_pmat + ipm;
                         ipm += align_up(jb.layout == kPTips ? (size_t)t->ncodes * CM * KP
-                                                            : (size_t)img_floats(KP, CM),
+                                                            : (size_t)img_words(KP, CM, ctx->prec),
                                         64);
                     }
                     oc.mat = jb.out;
@@ -1272,7 +1290,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         if (ln.nblk == 0)
             continue;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
-                            stack_depth(ln.K, ln.CM, ctx->depth4), st,
+                            trav_depth(ctx, ln.K, ln.CM), ln.K == 4 ? kPrecF32 : ctx->prec, st,
                             tm && i == first ? S.ev[2] : nullptr,
                             tm && i == last ? S.ev[3] : nullptr) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
@@ -1508,6 +1526,11 @@ int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value)
             return set_error(CRUX_EINVAL, "stack depth %d", value);
         ctx->depth4 = value;
         return CRUX_OK;
+    case CRUX_TUNE_PRECISION:
+        if (value < kPrecF32 || value > kPrecF16x3)
+            return set_error(CRUX_EINVAL, "precision %d", value);
+        ctx->prec = value;
+        return CRUX_OK;
     default:
         return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
     }
@@ -1520,6 +1543,9 @@ int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value)
     switch (key) {
     case CRUX_TUNE_STACK_DEPTH:
         *value = ctx->depth4;
+        return CRUX_OK;
+    case CRUX_TUNE_PRECISION:
+        *value = ctx->prec;
         return CRUX_OK;
     default:
         return set_error(CRUX_EINVAL, "unknown tuning key %d", key);

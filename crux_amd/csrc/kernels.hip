@@ -159,6 +159,44 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs, int
     }
     __syncthreads();
     const int cm = J.cm, kp = J.kp;
+    if (J.layout == kPImage && J.prec != kPrecF32) {
+        // 16-bit image: dword v of lane l (dword v % 4 of the lane's 16 bytes
+        // v / 4) holds P_c[4 I + i][(g cols + a) 4 + 2 p + e] in half e, for
+        // v = (I groups + g) 2 + p and l = c (64 / cm) + 4 a + i; the split
+        // precision's low image follows its high image.  P is rounded to
+        // FP32, then to 16 bits (round to nearest even).
+        const int G = 64 / cm, W = 16 / cm, NG = (kp + 4 * W - 1) / (4 * W), NV1 = (kp / 4) * NG * 2;
+        const int NV1P = (NV1 + 3) & ~3;
+        const int n = (J.prec == kPrecF16x3 ? 2 : 1) * NV1P * 64;
+        uint32_t *o = reinterpret_cast<uint32_t *>(J.out);
+        for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+            const int v = (idx >> 8) * 4 + (idx & 3), l = (idx >> 2) & 63;
+            uint32_t word = 0;
+            const int part = v / NV1P, vv = v - part * NV1P;
+            if (vv < NV1) {
+                const int pq = vv >> 1, I = pq / NG, g = pq - I * NG;
+                const int c = l / G, a = (l % G) >> 2, row = 4 * I + (l & 3);
+                const int col = (g * W + a) * 4 + 2 * (vv & 1);
+                for (int e = 0; e < 2; ++e) {
+                    float f = 0.0f;
+                    if (c < C && row < K && col + e < K)
+                        f = (float)P[(c * K + row) * K + col + e];
+                    uint32_t h = 0;
+                    if (J.prec == kPrecBF16) {
+                        const uint32_t u = __float_as_uint(f);
+                        h = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+                    } else {
+                        const _Float16 hi = (_Float16)f;
+                        const _Float16 x = part == 0 ? hi : (_Float16)(f - (float)hi);
+                        h = __builtin_bit_cast(uint16_t, x);
+                    }
+                    word |= h << (16 * e);
+                }
+            }
+            o[idx] = word;
+        }
+        return;
+    }
     if (J.layout == kPImage) {
         // Register v of lane l (float v % 4 of float4 v / 4 of the lane):
         // P_c[4 I + i][g * cols + a] for v = I * groups + g and
@@ -1190,7 +1228,7 @@ __host__ __device__ constexpr int trav20_waves()
     return KP == 20 && CM <= 4 ? 2 : 1;
 }
 
-template <int KP, int CM, int D>
+template <int KP, int CM, int D, int PREC>
 __global__ void __launch_bounds__(64, (trav20_waves<KP, CM>()))
 k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
@@ -1199,7 +1237,12 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     constexpr int CBSZ = W == 16 ? 4 : W == 8 ? 3 : W == 4 ? 2 : W == 2 ? 1 : 0;
     constexpr int NI = KP / 4;                      // row blocks = float4 per CLV row
     constexpr int NG = img_groups(KP, CM);
-    constexpr int NQ = img_floats(KP, CM) / 256;    // float4 per lane of a P image
+    constexpr int NG16 = img16_groups(KP, CM);      // 16-bit images: column groups
+    constexpr int NV16P = img16_nvp(KP, CM);        // dwords of one 16-bit image, padded
+    // float4 per lane of a prefetched P image (the split precision's low
+    // image is loaded where it is used, so that its registers stay within
+    // two waves per SIMD).
+    constexpr int NQ = PREC == kPrecF16x3 ? NV16P / 4 : img_words(KP, CM, PREC) / 256;
     constexpr int PQ = NQ > NI ? NQ : NI;           // float4 of a prefetched child
     constexpr int WPR = G / 4;                      // code dwords per staged tip row
     constexpr int RPI = 64 / WPR;                   // tip rows per staging load
@@ -1289,25 +1332,102 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         }
     };
 
-    // y = P_c x for every lane's own (site, component).
-    auto pmul = [&](const crux_f32x4 (&img)[PQ], const crux_f32x4 (&x)[NI], crux_f32x4 (&y)[NI]) {
+    // 16-bit products: four columns per instruction, the lane's own four
+    // states as B and, broadcast from block ABID of its component group,
+    // four columns of four P rows as A.
+    typedef short crux_s16x4 __attribute__((ext_vector_type(4)));
+    typedef _Float16 crux_f16x4 __attribute__((ext_vector_type(4)));
+    typedef unsigned int crux_u32x2 __attribute__((ext_vector_type(2)));
+    auto dword = [](const crux_f32x4 (&img)[PQ], int v) -> uint32_t {
+        return __float_as_uint(img[v >> 2][v & 3]);
+    };
+    auto bf16x4 = [](crux_f32x4 v) -> crux_s16x4 {
+        crux_s16x4 r;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t u = __float_as_uint(v[e]);
+            r[e] = (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+        }
+        return r;
+    };
+    auto pmul16 = [&](const crux_f32x4 (&img)[PQ], const crux_f32x4 (&x)[NI], crux_f32x4 (&y)[NI],
+                      const char *mat) {
+        crux_f32x4 lo[PREC == kPrecF16x3 ? NV16P / 4 : 1];
+        if constexpr (PREC == kPrecF16x3) {
+            const auto rl = rsrc_of(mat);
+#pragma unroll
+            for (int q = 0; q < NV16P / 4; ++q)
+                lo[q] = bload4(rl, (uint32_t)lane * 16, (uint32_t)(NV16P / 4 + q) * 1024);
+        }
+        auto loword = [&](int v) -> uint32_t { return __float_as_uint(lo[v >> 2][v & 3]); };
 #pragma unroll
         for (int I = 0; I < NI; ++I)
             y[I] = crux_f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            static_for<W>([&](auto A) {
-                constexpr int a = decltype(A)::value;
-                const int col = g * W + a;
-                if (col < KP) {
+        for (int g = 0; g < NG16; ++g) {
+            static_for<W>([&](auto Ac) {
+                constexpr int a = decltype(Ac)::value;
+                const int q = g * W + a;             // float4 of states (4 columns)
+                if (q < NI) {
+                    const crux_f32x4 xv = x[q];
+                    if constexpr (PREC == kPrecBF16) {
+                        const crux_s16x4 bx = bf16x4(xv);
 #pragma unroll
-                    for (int I = 0; I < NI; ++I) {
-                        const int v = I * NG + g;
-                        y[I] = __builtin_amdgcn_mfma_f32_4x4x1f32(img[v >> 2][v & 3], x[col >> 2][col & 3],
-                                                                  y[I], CBSZ, a, 0);
+                        for (int I = 0; I < NI; ++I) {
+                            const int v = (I * NG16 + g) * 2;
+                            const crux_s16x4 ap = __builtin_bit_cast(
+                                crux_s16x4, crux_u32x2{dword(img, v), dword(img, v + 1)});
+                            y[I] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(ap, bx, y[I], CBSZ, a, 0);
+                        }
+                    } else {
+                        const crux_f16x4 hx = __builtin_convertvector(xv, crux_f16x4);
+                        crux_f16x4 lx;
+                        if constexpr (PREC == kPrecF16x3)
+                            lx = __builtin_convertvector(xv - __builtin_convertvector(hx, crux_f32x4), crux_f16x4);
+#pragma unroll
+                        for (int I = 0; I < NI; ++I) {
+                            const int v = (I * NG16 + g) * 2;
+                            const crux_f16x4 ah = __builtin_bit_cast(
+                                crux_f16x4, crux_u32x2{dword(img, v), dword(img, v + 1)});
+                            if constexpr (PREC == kPrecF16x3) {
+                                const crux_f16x4 al = __builtin_bit_cast(
+                                    crux_f16x4, crux_u32x2{loword(v), loword(v + 1)});
+                                // The small terms first, then the main one.
+                                y[I] = __builtin_amdgcn_mfma_f32_4x4x4f16(al, hx, y[I], CBSZ, a, 0);
+                                y[I] = __builtin_amdgcn_mfma_f32_4x4x4f16(ah, lx, y[I], CBSZ, a, 0);
+                            }
+                            y[I] = __builtin_amdgcn_mfma_f32_4x4x4f16(ah, hx, y[I], CBSZ, a, 0);
+                        }
                     }
                 }
             });
+        }
+    };
+
+    // y = P_c x for every lane's own (site, component).
+    auto pmul = [&](const crux_f32x4 (&img)[PQ], const crux_f32x4 (&x)[NI], crux_f32x4 (&y)[NI],
+                    const char *mat) {
+        if constexpr (PREC != kPrecF32) {
+            pmul16(img, x, y, mat);
+        } else {
+#pragma unroll
+            for (int I = 0; I < NI; ++I)
+                y[I] = crux_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                static_for<W>([&](auto A) {
+                    constexpr int a = decltype(A)::value;
+                    const int col = g * W + a;
+                    if (col < KP) {
+#pragma unroll
+                        for (int I = 0; I < NI; ++I) {
+                            const int v = I * NG + g;
+                            y[I] = __builtin_amdgcn_mfma_f32_4x4x1f32(img[v >> 2][v & 3], x[col >> 2][col & 3],
+                                                                      y[I], CBSZ, a, 0);
+                        }
+                    }
+                });
+            }
         }
     };
 
@@ -1395,7 +1515,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                         for (int I = 0; I < NI; ++I)
                             y[I] = c < C ? x[I] : crux_f32x4{0.f, 0.f, 0.f, 0.f};
                     } else {
-                        pmul(pj, x, y);
+                        pmul(pj, x, y, ptr(cw0, 8 + 6 * j));
                     }
                 }
 #pragma unroll
@@ -1506,17 +1626,17 @@ static void launch_trav4(const Entry *entries, const int32_t *block_entry, int b
                               t1, 0, entries, block_entry, blk0);
 }
 
-template <int KP, int CM, int D>
+template <int KP, int CM, int D, int PREC>
 static void launch_trav20(const Entry *entries, const int32_t *block_entry, int blk0,
                           int nblocks, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
-    hipExtLaunchKernelGGL((k_trav20<KP, CM, D>), dim3(nblocks), dim3(64), 0, stream, t0, t1, 0,
+    hipExtLaunchKernelGGL((k_trav20<KP, CM, D, PREC>), dim3(nblocks), dim3(64), 0, stream, t0, t1, 0,
                           entries, block_entry, blk0);
 }
 
 template <int K, int CM>
 static void launch_traverse(const Entry *entries, const int32_t *block_entry, int blk0,
-                            int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
+                            int nblocks, int depth, int prec, hipStream_t stream, hipEvent_t t0,
                             hipEvent_t t1)
 {
     // Stack depth: more slots keep more results in LDS, fewer let more
@@ -1530,12 +1650,19 @@ static void launch_traverse(const Entry *entries, const int32_t *block_entry, in
             launch_trav4<CM, 8>(entries, block_entry, blk0, nblocks, stream, t0, t1);
     } else {
         constexpr int KP = kpad(K);
-        if (depth <= 2)
-            launch_trav20<KP, CM, 2>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        // 16-bit inputs are instantiated at the default stack depth only.
+        if (prec == kPrecBF16)
+            launch_trav20<KP, CM, 4, kPrecBF16>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (prec == kPrecF16)
+            launch_trav20<KP, CM, 4, kPrecF16>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (prec == kPrecF16x3)
+            launch_trav20<KP, CM, 4, kPrecF16x3>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (depth <= 2)
+            launch_trav20<KP, CM, 2, kPrecF32>(entries, block_entry, blk0, nblocks, stream, t0, t1);
         else if (depth <= 4)
-            launch_trav20<KP, CM, 4>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+            launch_trav20<KP, CM, 4, kPrecF32>(entries, block_entry, blk0, nblocks, stream, t0, t1);
         else
-            launch_trav20<KP, CM, 8>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+            launch_trav20<KP, CM, 8, kPrecF32>(entries, block_entry, blk0, nblocks, stream, t0, t1);
     }
 }
 
@@ -1558,13 +1685,13 @@ int traverse_cmax(int K, int C)
 }
 
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, int depth, hipStream_t stream, hipEvent_t t0,
+                    int blk0, int nblocks, int depth, int prec, hipStream_t stream, hipEvent_t t0,
                     hipEvent_t t1)
 {
-#define CRUX_CASE(KK, CC)                                                                 \
-    if (K == KK && CM == CC) {                                                            \
-        launch_traverse<KK, CC>(entries, block_entry, blk0, nblocks, depth, stream, t0, t1); \
-        return 0;                                                                         \
+#define CRUX_CASE(KK, CC)                                                                       \
+    if (K == KK && CM == CC) {                                                                  \
+        launch_traverse<KK, CC>(entries, block_entry, blk0, nblocks, depth, prec, stream, t0, t1); \
+        return 0;                                                                               \
     }
     CRUX_CASE(4, 1) CRUX_CASE(4, 2) CRUX_CASE(4, 4) CRUX_CASE(4, 8) CRUX_CASE(4, 16)
     CRUX_CASE(20, 1) CRUX_CASE(20, 2) CRUX_CASE(20, 4) CRUX_CASE(20, 8)

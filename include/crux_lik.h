@@ -207,8 +207,15 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  *   CRUX_TUNE_STACK_DEPTH: LDS stack slots per lane of the traversal
  *   kernels (2, 4 or 8, capped by what fits; 0 = default: 8 for 4 states
  *   with up to 2 components, else 4).  Fewer slots let more blocks share a
- *   CU; more slots keep more pending results out of HBM. */
-enum { CRUX_TUNE_STACK_DEPTH = 1 };
+ *   CU; more slots keep more pending results out of HBM.
+ *   CRUX_TUNE_PRECISION: input precision of the 20/22-state P * CLV
+ *   products on the matrix core, accumulated in FP32 (CLVs are stored in
+ *   FP32 either way): CRUX_PREC_F32 (default), CRUX_PREC_BF16, CRUX_PREC_F16,
+ *   or CRUX_PREC_F16X3 (float16 high and low parts, three products).  The
+ *   16-bit forms run at stack depth 4.  Likelihoods evaluated at another
+ *   precision recompute their CLVs. */
+enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
+enum { CRUX_TUNE_STACK_DEPTH = 1, CRUX_TUNE_PRECISION = 2 };
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
 
