@@ -260,6 +260,7 @@ struct PlanOp {
     ChildList ch;
     int32_t from_stack = 0;            // bit j: child j comes from the LDS stack
     int32_t fwd = 0;                   // bit j: child j is the previous update's result
+    int32_t own = 0;                   // bit j: child j is this plan's result, re-read from HBM
     int8_t pos[kOpMaxChild + 1] = {-1, -1, -1, -1};
     int32_t tip_load = -1;             // tip chunk staged before this op (entry-relative)
     int32_t tip_first = 0;             // index of the op's first tip in the evaluation's list
@@ -587,7 +588,11 @@ struct Planner {
 // identity child, else 0.
 static int op_shape(const Op &op, const PlanOp &pop)
 {
-    if (op.nchild != 2 || pop.parent_slot < 0)
+    // Results of this plan that did not fit the stack are re-read from HBM
+    // where they are used (the generic walk waits for every store first);
+    // the shapes' HBM children are CLVs cached by earlier plans, which the
+    // 4-state kernel loads one update ahead.
+    if (op.nchild != 2 || pop.parent_slot < 0 || pop.own)
         return 0;
     int src[2];
     for (int j = 0; j < 2; j++) {
@@ -744,6 +749,7 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
         PlanOp op = ops[i];
         op.from_stack = 0;
         op.fwd = 0;
+        op.own = 0;
         for (int j = 0; j < kOpMaxChild; j++)
             op.pos[j] = -1;
         int popped = 0;
@@ -755,6 +761,8 @@ static void schedule_stack(std::vector<PlanOp> &ops, int depth)
                 op.from_stack |= 1 << j;
                 op.pos[j] = (int8_t)push_slot[p];
                 popped++;
+            } else if (p >= 0) {
+                op.own |= 1 << j;
             }
         }
         sp -= popped;

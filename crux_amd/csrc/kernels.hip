@@ -885,6 +885,14 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 #pragma unroll
     for (int m = 0; m < CL; ++m)
         yl[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // Children of the next update cached in HBM by an earlier plan (partial
+    // re-evaluations), loaded while the current update computes and before
+    // it issues its stores: the next update then waits for these loads only,
+    // not for the current update's stores as well (one counter tracks both,
+    // in order).  npf: they were loaded (not for a chunk's first update).
+    float4 xp[2][CL];
+    int32_t epf[2] = {0, 0};
+    bool npf = false;
 
     for (int k = 0; k < nchunks; ++k) {
     const int buf = k & 1, o0 = k * PC, n = min(PC, nops - o0);
@@ -942,8 +950,8 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         };
         // A child cached by an earlier plan (partial re-evaluation): planes
         // past the slot's C read as zero (buffer range).  The caller waits.
-        auto load_child = [&](int j, float4 (&x)[CL]) -> int32_t {
-            const CRUX_CAS OpChild &ch = op->ch[j];
+        auto load_child_of = [&](const CRUX_CAS Op *uop, int j, float4 (&x)[CL]) -> int32_t {
+            const CRUX_CAS OpChild &ch = uop->ch[j];
             const char *cb = reinterpret_cast<const char *>(ch.clv);
             const auto rc = rsrc_of(cb, (uint32_t)E->exp_off);
 #pragma unroll
@@ -953,6 +961,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             }
             return bload1(rsrc_of(ch.exps), (uint32_t)sc * 4, 0);
         };
+        auto load_child = [&](int j, float4 (&x)[CL]) -> int32_t { return load_child_of(op, j, x); };
         // Two-child updates with a parent and no identity child (most
         // updates) run a straight-line body specialised for their children's
         // sources; the control word names the pair.  acc = d0 * d1 and the
@@ -969,12 +978,24 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 read_tip(1);
             float4 xh[2][CL];
             int32_t eh[2] = {0, 0};
-            if constexpr (s0 == kSrcHbm)
-                eh[0] = load_child(0, xh[0]);
-            if constexpr (s1 == kSrcHbm)
-                eh[1] = load_child(1, xh[1]);
-            if constexpr (s0 == kSrcHbm || s1 == kSrcHbm)
-                __builtin_amdgcn_s_waitcnt(0);
+            if constexpr (s0 == kSrcHbm || s1 == kSrcHbm) {
+                if (npf) {
+                    // Prefetched by the previous update.
+#pragma unroll
+                    for (int m = 0; m < CL; ++m) {
+                        xh[0][m] = xp[0][m];
+                        xh[1][m] = xp[1][m];
+                    }
+                    eh[0] = epf[0];
+                    eh[1] = epf[1];
+                } else {
+                    if constexpr (s0 == kSrcHbm)
+                        eh[0] = load_child(0, xh[0]);
+                    if constexpr (s1 == kSrcHbm)
+                        eh[1] = load_child(1, xh[1]);
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+            }
             crux_f32x4 d[2][CL];
             static_for<2>([&](auto Jc) {
                 constexpr int j = decltype(Jc)::value;
@@ -1074,9 +1095,21 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                     acc[m] = (j == 0) ? v[m] : mul4(acc[m], v[m]);
             });
         }
-        // The next update's control words and P image, while this one ends.
-        if (oq + 1 < n)
+        // The next update's control words and P image, while this one ends,
+        // and the children it takes from HBM (two-child shapes).
+        npf = false;
+        if (oq + 1 < n) {
             read_op(buf, oq + 1);
+            const int ns = (__builtin_amdgcn_readfirstlane(cwa.x) >> 25) & 31;
+            if (ns != 0) {
+                const CRUX_CAS Op *nop = op + 1;
+                if (((ns - 1) >> 2) == kSrcHbm)
+                    epf[0] = load_child_of(nop, 0, xp[0]);
+                if (((ns - 1) & 3) == kSrcHbm)
+                    epf[1] = load_child_of(nop, 1, xp[1]);
+                npf = true;
+            }
+        }
         // Site maximum over the lane's components, then over the halves, and
         // the rescaled result (also formed at the root, where nothing uses
         // it: the loop-carried registers then have one definition).
