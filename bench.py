@@ -459,6 +459,7 @@ def bench_serve(args):
         s["mean_batch"] = round((srv.batched_requests - q0) / max(1, srv.batches - b0), 1)
         log("serve rate %g: %s" % (r, s))
         sweep.append(s)
+    srv.close()
     return {"workload": "served likelihood: Poisson arrivals, 70% single-branch / 30% full, "
                         "16 resident trees of 32-1,024 taxa x 500-200k sites, DNA and 20-state "
                         "amino acids 1:1, GTR+G4",

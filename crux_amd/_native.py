@@ -57,6 +57,15 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_uint64)]),
     "crux_batch_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, _dp]),
     "crux_batch_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "crux_server_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "crux_server_destroy": (ctypes.c_int, [_vp]),
+    "crux_server_add_lik": (ctypes.c_int, [_vp, _vp]),
+    "crux_server_remove_lik": (ctypes.c_int, [_vp, _vp]),
+    "crux_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _i32p, _dp,
+                                          ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "crux_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "crux_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _dp, ctypes.POINTER(ctypes.c_int64)]),
+    "crux_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "crux_ctx_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "crux_ctx_get_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcrux_amd.so")
-SOURCES = ["engine.hip", "kernels.hip", "model.cpp"]
+SOURCES = ["engine.hip", "kernels.hip", "model.cpp", "server.cpp"]
 ARCH = os.environ.get("CRUX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -282,6 +282,7 @@ struct crux_tree {
     std::vector<RingCache> cache[2];
     Arena arena[2];
     crux_lik *owner[2] = {nullptr, nullptr};
+    std::vector<crux_lik *> liks;   // every likelihood created on the tree
 
     size_t mask_stride() const { return align_up((size_t)S, kTile4); }
     // One power-of-two exponent per site, shared by all components.
@@ -1669,9 +1670,17 @@ int crux_tree_destroy(crux_tree *t)
     dev_free(ctx, t->d_codevecs, sizeof(double) * t->ncodes * t->K);
     dev_free(ctx, t->d_masks, (size_t)t->ntaxa * t->mask_stride());
     dev_free(ctx, t->d_weights, sizeof(uint32_t) * t->S);
-    for (int p = 0; p < 2; p++)
-        if (t->owner[p])
-            t->owner[p]->tree = nullptr;
+    // Likelihoods still alive lose their device buffers and their tree
+    // (crux_lik_destroy then only frees the handle).
+    for (crux_lik *L : t->liks) {
+        dev_free(ctx, L->d_model, L->model_bytes);
+        dev_free(ctx, L->d_piw, sizeof(double) * L->C * t->K);
+        dev_free(ctx, L->d_site, sizeof(double) * t->S);
+        L->d_model = nullptr;
+        L->d_piw = nullptr;
+        L->d_site = nullptr;
+        L->tree = nullptr;
+    }
     delete t;
     bool last;
     {
@@ -1844,6 +1853,10 @@ int crux_lik_create(crux_tree *t, int polarity, crux_lik **out)
         delete L;
         return rc;
     }
+    {
+        std::lock_guard<std::mutex> g(t->ctx->mu);
+        t->liks.push_back(L);
+    }
     *out = L;
     return CRUX_OK;
 }
@@ -1865,6 +1878,10 @@ int crux_lik_destroy(crux_lik *L)
             t->owner[L->pol] = nullptr;
         if (L->root_tmp_slot >= 0 && L->root_tmp_gen == t->arena[L->pol].gen)
             arena_put(t, L->pol, L->root_tmp_slot);
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            t->liks.erase(std::remove(t->liks.begin(), t->liks.end(), L), t->liks.end());
+        }
         dev_free(ctx, L->d_model, L->model_bytes);
         dev_free(ctx, L->d_piw, sizeof(double) * L->C * t->K);
         dev_free(ctx, L->d_site, sizeof(double) * t->S);
@@ -2039,6 +2056,84 @@ int crux_batch_poll(crux_ctx *ctx, uint64_t ticket, int *done)
     if (!ctx || !done)
         return set_error(CRUX_EINVAL, "bad arguments");
     std::lock_guard<std::mutex> g(ctx->mu);
+    return crux::poll_locked(ctx, ticket, done);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches assembled one likelihood at a time (used by the serving loop).
+
+namespace crux {
+
+void ctx_lock(crux_ctx *ctx) { ctx->mu.lock(); }
+void ctx_unlock(crux_ctx *ctx) { ctx->mu.unlock(); }
+
+int plan_locked(crux_lik *L, int root)
+{
+    if (!L || !L->tree)
+        return set_error(CRUX_EINVAL, "bad lik");
+    return plan_lik(L, root);
+}
+
+void flush_locked(crux_lik *L)
+{
+    crux_tree *t = L->tree;
+    if (t->owner[L->pol] == L) {
+        for (RingCache &c : t->cache[L->pol]) {
+            if (c.slot >= 0) {
+                arena_put(t, L->pol, c.slot);
+                c.slot = -1;
+            }
+            c.valid = false;
+            c.parent = INT32_MIN;
+        }
+    }
+    L->root_valid = false;
+}
+
+int submit_planned(crux_ctx *ctx, crux_lik *const *liks, int n, uint64_t *ticket)
+{
+    std::vector<crux_lik *> v(liks, liks + n);
+    int rc = submit(ctx, v, ticket);
+    if (rc)
+        for (crux_lik *L : v)
+            L->invalidate = true;
+    return rc;
+}
+
+int collect_locked(crux_ctx *ctx, uint64_t ticket, double *out, size_t n)
+{
+    return collect(ctx, ticket, out, n);
+}
+
+void tree_release_clvs(crux_tree *t)
+{
+    crux_ctx *ctx = t->ctx;
+    // Batches in flight may still write this tree's slots.
+    (void)drain(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int p = 0; p < 2; p++) {
+        arena_reset(t, p);
+        if (t->owner[p])
+            t->owner[p]->root_valid = false;
+    }
+}
+
+uint64_t tree_clv_bytes(const crux_tree *t)
+{
+    uint64_t b = 0;
+    for (int p = 0; p < 2; p++)
+        b += (uint64_t)t->arena[p].chunks.size() * t->arena[p].chunk_slots * t->arena[p].slot_bytes;
+    return b;
+}
+
+crux_tree *lik_tree(const crux_lik *L) { return L->tree; }
+crux_ctx *tree_ctx(const crux_tree *t) { return t->ctx; }
+int lik_polarity(const crux_lik *L) { return L->pol; }
+
+int poll_locked(crux_ctx *ctx, uint64_t ticket, int *done)
+{
     *done = 0;
     if (ctx->results.count(ticket)) {
         *done = 1;
@@ -2064,6 +2159,10 @@ int crux_batch_poll(crux_ctx *ctx, uint64_t ticket, int *done)
     return set_error(CRUX_EINVAL, "unknown or collected batch ticket %llu",
                      (unsigned long long)ticket);
 }
+
+}  // namespace crux
+
+extern "C" {
 
 int crux_lik_lnl_noalloc(crux_lik *L, int root, double *lnl)
 {

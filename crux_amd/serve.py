@@ -1,4 +1,5 @@
-"""Continuous-batching likelihood server over one device context.
+"""Served likelihood: a Poisson stream of requests against resident trees,
+driven through the native continuous-batching server (crux_server_*).
 
 Requests arrive over time (Poisson), each naming a resident tree (a
 session): 70% single-branch re-evaluations (one edge length changed by an
@@ -6,15 +7,15 @@ MC3-style multiplier, lnL rooted at that edge) and 30% full-tree lnLs (cache
 flushed, every CLV recomputed).  Trees, alignments, CLV caches and model
 eigensystems stay on the device between requests.
 
-The loop keeps up to two fused batches in flight (``crux_batch_submit`` /
-``crux_batch_wait``): while the device runs one batch, every request that
-has arrived is admitted into the next one, as long as its session is not
-already in that batch and its tree fits the HBM budget (sessions not in
-flight are evicted least-recently-used to make room; a request whose tree
-cannot fit waits).  Each batch is planned and uploaded while the previous
-one runs, and batches run back to back on the device in submission order,
-so the requests of one session are served in arrival order.  Reported:
-throughput and p50/p99 latency (arrival to result) per arrival rate.
+Batching, ordering and admission are the native server's: a server thread
+forms a batch from every request that has arrived while the device runs the
+previous one (up to two batches in flight), up to two requests of a tree per
+batch (one per CLV polarity: each session serves through a likelihood and
+its clone), each planned against the tree as the requests before it leave
+it; the context's HBM budget is enforced by releasing the CLV pools of idle
+trees, least recently used first.  This module submits each request at its
+arrival time and reports throughput and p50/p99 latency (arrival to result)
+per arrival rate.
 
     python -m crux_amd.serve [--sessions 16] [--requests 2000] [--rates 2000,8000,...]
 """
@@ -22,7 +23,7 @@ throughput and p50/p99 latency (arrival to result) per arrival rate.
 from __future__ import annotations
 
 import argparse
-import collections
+import ctypes
 import json
 import math
 import sys
@@ -38,21 +39,21 @@ BRLEN_LAMBDA = 2.0 * math.log(1.6)
 
 
 class Session:
-    """A resident tree: problem, device Lik, and its request bookkeeping."""
+    """A resident tree: problem, a device Lik and its clone (the two CLV
+    polarities the server may use)."""
 
     def __init__(self, sid, prob, ncat=4):
         self.sid = sid
         self.prob = prob
         self.ncat = ncat
         self.lik = None
+        self.mate = None
         self.tree = None
         self.edges = None
-        self.last_used = 0.0
-        self.in_flight = 0
 
     def footprint(self, C=4):
-        """Device bytes of a fully cached tree (both CLV directions of every
-        internal edge, FP32, plus exponents, tips and site buffers)."""
+        """Device bytes of a fully cached tree in one polarity (CLVs of every
+        directed internal edge, FP32, plus exponents, tips and site buffers)."""
         aln = self.prob.alignment
         S, K, N = aln.nchars, 4 if self.prob.kind == "dna" else 20, aln.ntaxa
         slots = 2 * (2 * N - 3)
@@ -63,22 +64,27 @@ class Session:
         return self.lik is not None
 
     def load(self, ctx):
-        # The host tree outlives evictions: it carries the session's state
-        # (branch lengths edited by earlier requests).
+        # The host tree outlives the device objects: it carries the session's
+        # state (branch lengths edited by earlier requests).
         if self.tree is None:
             self.tree = self.prob.fresh_tree()
             self.edges = list(self.tree.edges)
         self.lik = Lik(self.tree, self.prob.alignment, ncat=self.ncat, context=ctx)
         synth.configure(self.lik, self.prob.model)
+        self.mate = self.lik.clone()
+        # Models and topology on the device before the server takes over.
+        self.lik._ready(None)
+        self.mate._ready(None)
 
     def evict(self):
-        """Free the device state (CLV cache, tree mirror, model)."""
-        # The lik first (it refers to the device tree), then the tree.
-        if self.lik._handle:
-            nat.lib().crux_lik_destroy(self.lik._handle)
-            self.lik._handle = None
+        """Free the device state (CLV caches, tree mirror, models)."""
+        for L in (self.mate, self.lik):
+            if L is not None and L._handle:
+                nat.lib().crux_lik_destroy(L._handle)
+                L._handle = None
         self.lik._dev.close()
         self.lik = None
+        self.mate = None
 
 
 class Request:
@@ -124,100 +130,98 @@ def make_requests(sessions, n, rate, seed=6, p_branch=0.7):
 
 
 def _apply(sess, req):
-    """Mutate the session for the request; return the node to root at."""
+    """Apply the request to the host tree; return (root node, edited edge ids,
+    their new lengths, flush)."""
     if req.kind == "branch":
         e = sess.edges[req.edge % len(sess.edges)]
         e.length = e.length * req.mult
-        return e.ring.node
-    sess.lik.flush()
-    return sess.tree.base
+        return e.ring.node, [e.id], [e.length], False
+    return sess.tree.base, [], [], True
 
 
 class Server:
-    def __init__(self, sessions, ctx=None, hbm_budget=None, max_batch=256):
+    """The native continuous-batching server on one context."""
+
+    def __init__(self, sessions, ctx=None, max_batch=256):
         self.ctx = ctx if ctx is not None else nat.get_context(0)
         self.sessions = {s.sid: s for s in sessions}
-        self.budget = hbm_budget
-        self.max_batch = max_batch
-        self.batches = 0
-        self.batched_requests = 0
-        self.evictions = 0
+        h = nat._vp()
+        nat.check(nat.lib().crux_server_create(self.ctx.handle, int(max_batch), ctypes.byref(h)))
+        self.handle = h
+        self._added = set()
 
-    def _fits(self, sess):
-        """Admission by HBM budget: make room for a tree that is not resident
-        by evicting idle sessions, least recently used first."""
-        if self.budget is None or sess.resident:
-            return True
-        # Each resident tree is charged its fully cached footprint, so a
-        # tree's CLV cache can grow to all its directed edges within budget.
-        need = sess.footprint()
-        while sum(s.footprint() for s in self.sessions.values() if s.resident) + need > self.budget:
-            idle = [s for s in self.sessions.values() if s.resident and s.in_flight == 0]
-            if not idle:
-                # Nothing left to evict: wait for in-flight work, unless the
-                # device holds no tree at all (a tree above the budget then
-                # runs alone rather than never).
-                return not any(s.resident for s in self.sessions.values())
-            min(idle, key=lambda s: s.last_used).evict()
-            self.evictions += 1
-        return True
+    def _ensure(self, sess):
+        if not sess.resident:
+            sess.load(self.ctx)
+        if sess.sid not in self._added:
+            for L in (sess.lik, sess.mate):
+                nat.check(nat.lib().crux_server_add_lik(self.handle, L.handle))
+            self._added.add(sess.sid)
 
-    def run(self, requests, clock=None):
-        """Serve requests (sorted by arrival time, seconds from start) in
-        real time.  Returns the requests with t_done and lnl filled in."""
-        clock = clock or time.perf_counter
-        t0 = clock()
-        arrivals = collections.deque(requests)
-        queue = collections.deque()
-        inflight = collections.deque()   # (ticket, [requests])
-        done = 0
-        n = len(requests)
-        while done < n:
-            now = clock() - t0
-            while arrivals and arrivals[0].t_arrive <= now:
-                queue.append(arrivals.popleft())
-            if queue and len(inflight) < 2:
-                batch, seen, rest = [], set(), collections.deque()
-                while queue and len(batch) < self.max_batch:
-                    r = queue.popleft()
-                    sess = self.sessions[r.sid]
-                    if r.sid in seen or not self._fits(sess):
-                        rest.append(r)
-                        continue
-                    seen.add(r.sid)
-                    if not sess.resident:
-                        sess.load(self.ctx)
-                    # Busy from now on: a later admission may not evict it.
-                    sess.in_flight += 1
-                    batch.append(r)
-                queue.extendleft(reversed(rest))
-                if batch:
-                    liks, roots = [], []
-                    for r in batch:
-                        sess = self.sessions[r.sid]
-                        node = _apply(sess, r)
-                        liks.append(sess.lik)
-                        roots.append(sess.lik._ready(node))
-                        sess.last_used = now
-                    inflight.append((nat.batch_submit(self.ctx, liks, roots), batch))
-                    self.batches += 1
-                    self.batched_requests += len(batch)
-                    continue
-            if inflight:
-                ticket, batch = inflight.popleft()
-                out = nat.batch_wait(self.ctx, ticket)
-                t = clock() - t0
-                for r, v in zip(batch, out):
-                    r.lnl = float(v)
-                    r.t_done = t
-                    self.sessions[r.sid].in_flight -= 1
-                done += len(batch)
-                continue
-            if arrivals:
-                dt = arrivals[0].t_arrive - (clock() - t0)
-                if dt > 0:
-                    time.sleep(min(dt, 0.001))
+    def stats(self):
+        """(batches, requests batched, trees released for the HBM budget,
+        failed requests)."""
+        st = (ctypes.c_uint64 * 4)()
+        nat.check(nat.lib().crux_server_stats(self.handle, st))
+        return tuple(int(x) for x in st)
+
+    @property
+    def batches(self):
+        return self.stats()[0]
+
+    @property
+    def batched_requests(self):
+        return self.stats()[1]
+
+    @property
+    def evictions(self):
+        return self.stats()[2]
+
+    def run(self, requests):
+        """Submit each request at its arrival time (seconds from now), then
+        collect every result.  Returns the requests with lnl and t_done
+        (seconds from the start) filled in."""
+        lib = nat.lib()
+        for s in {r.sid for r in requests}:
+            self._ensure(self.sessions[s])
+        ids = []
+        t0 = time.perf_counter_ns()
+        out = ctypes.c_uint64()
+        for r in requests:
+            while (time.perf_counter_ns() - t0) * 1e-9 < r.t_arrive:
+                pass
+            sess = self.sessions[r.sid]
+            node, edges, lens, flush = _apply(sess, r)
+            n = len(edges)
+            nat.check(lib.crux_server_submit(
+                self.handle, sess.lik.handle, node.id, n, (ctypes.c_int32 * max(1, n))(*edges),
+                (ctypes.c_double * max(1, n))(*lens), 1 if flush else 0, ctypes.byref(out)))
+            ids.append(out.value)
+        v = ctypes.c_double()
+        td = ctypes.c_int64()
+        for r, i in zip(requests, ids):
+            nat.check(lib.crux_server_wait(self.handle, i, ctypes.byref(v), ctypes.byref(td)))
+            r.lnl = v.value
+            r.t_done = (td.value - t0) * 1e-9
         return requests
+
+    def close(self):
+        """Stop the server (after its requests) and free the sessions."""
+        if self.handle:
+            for sid in list(self._added):
+                sess = self.sessions[sid]
+                if sess.resident:
+                    for L in (sess.lik, sess.mate):
+                        nat.lib().crux_server_remove_lik(self.handle, L.handle)
+            nat.lib().crux_server_destroy(self.handle)
+            self.handle = None
+            self._added.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def replay_alone(sessions, requests, ctx=None):
@@ -230,7 +234,9 @@ def replay_alone(sessions, requests, ctx=None):
         sess = fresh[r.sid]
         if not sess.resident:
             sess.load(ctx)
-        node = _apply(sess, r)
+        node, _, _, flush = _apply(sess, r)
+        if flush:
+            sess.lik.flush()
         out.append(sess.lik.lnL(node))
     for s in fresh.values():
         if s.resident:
@@ -254,19 +260,18 @@ def main(argv=None):
     ap.add_argument("--taxa", default="32,1024")
     ap.add_argument("--sites", default="500,200000")
     ap.add_argument("--budget-gib", type=float, default=0.0,
-                    help="HBM budget for resident trees (0: none)")
+                    help="HBM budget of the serving context (0: none)")
     args = ap.parse_args(argv)
     taxa = tuple(int(x) for x in args.taxa.split(","))
     sites = tuple(int(x) for x in args.sites.split(","))
     t0 = time.time()
     sessions = make_sessions(args.sessions, taxa=taxa, sites=sites)
     print("sessions: %d (%.1fs)" % (len(sessions), time.time() - t0), file=sys.stderr, flush=True)
-    ctx = nat.get_context(0)
-    budget = int(args.budget_gib * (1 << 30)) or None
-    srv = Server(sessions, ctx, hbm_budget=budget)
+    budget = int(args.budget_gib * (1 << 30))
+    ctx = nat.Context(0, budget) if budget else nat.get_context(0)
+    srv = Server(sessions, ctx)
     # Warm: every session resident with its cache built.
-    warm = make_requests(sessions, 4 * len(sessions), 1e9, seed=1)
-    srv.run(warm)
+    srv.run(make_requests(sessions, 4 * len(sessions), 1e9, seed=1))
     lines = []
     for rate in [float(x) for x in args.rates.split(",")]:
         reqs = make_requests(sessions, args.requests, rate, seed=int(rate) % 1000 + 7)
@@ -283,6 +288,7 @@ def main(argv=None):
                                   "DNA/AA 1:1, GTR+G4), 70%% single-branch / 30%% full"
                                   % (len(sessions), args.taxa, args.sites),
                       "sweep": lines}), flush=True)
+    srv.close()
     return 0
 
 

@@ -196,6 +196,47 @@ int crux_batch_wait(crux_ctx *ctx, uint64_t ticket, int n, double *out);
  * CxLikExecute (CxLik.c:914-968) with a poll. */
 int crux_batch_poll(crux_ctx *ctx, uint64_t ticket, int *done);
 
+/* ---- serving loop -------------------------------------------------------- */
+
+/* Continuous batching of likelihood requests on one context (no counterpart
+ * in the reference, whose callers evaluate one proposal per synchronous
+ * Lik.lnL call, pkg/Crux/Mc3/Chain.pyx:455).  A server thread forms a batch
+ * from every request that has arrived while the device runs the previous
+ * one (up to two batches in flight) and runs it as one fused device pass.
+ * Requests of one tree run in submission order; a batch takes up to two
+ * requests of a tree, one per polarity of the likelihoods added for it (a
+ * likelihood and its clone, which must hold the same model), each planned
+ * against the tree as the requests before it leave it.  Admission follows
+ * the context's HBM budget: when a request needs memory the budget does not
+ * allow, the CLV pools of the least recently used tree that is neither in
+ * flight nor in the batch being formed are released (recomputed when next
+ * used) and planning is retried; a request that still
+ * does not fit waits for the batches in flight, and fails with CRUX_ENOMEM
+ * only when nothing else holds memory.  While a likelihood is added, its
+ * tree is edited and evaluated by the server only. */
+typedef struct crux_server crux_server;
+int crux_server_create(crux_ctx *ctx, int max_batch, crux_server **out);
+/* Finishes every submitted request, then stops the server thread. */
+int crux_server_destroy(crux_server *srv);
+int crux_server_add_lik(crux_server *srv, crux_lik *lik);
+/* Waits for the tree's requests to finish, then forgets the likelihood. */
+int crux_server_remove_lik(crux_server *srv, crux_lik *lik);
+/* Request: set the lengths of nedges edges (edges[i] gets lens[i]), drop
+ * the likelihood's cached CLVs when flush != 0 (a full-tree evaluation),
+ * then evaluate lnL rooted at root_node (the request may be served by the
+ * tree's other added likelihood).  *req names it for poll and wait. */
+int crux_server_submit(crux_server *srv, crux_lik *lik, int root_node, int nedges,
+                       const int32_t *edges, const double *lens, int flush, uint64_t *req);
+/* *done = 1 once the request has finished (never blocks). */
+int crux_server_poll(crux_server *srv, uint64_t req, int *done);
+/* Blocks until the request has finished; returns its status and, when it
+ * succeeded, its lnL and completion time (CLOCK_MONOTONIC, ns).  Each
+ * request is collected once. */
+int crux_server_wait(crux_server *srv, uint64_t req, double *lnl, int64_t *t_done_ns);
+/* [0] batches, [1] requests batched, [2] trees whose CLVs were released for
+ * the HBM budget, [3] failed requests. */
+int crux_server_stats(crux_server *srv, uint64_t *stats4);
+
 /* Execution switches of the context (bench and tests):
  *   mode 0: fused (default) -- one traversal launch per batch;
  *   mode 1: plain -- one launch per node update plus one for the root,

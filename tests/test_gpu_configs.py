@@ -92,6 +92,7 @@ def test_served_mix_16_sessions_full_size():
     srv.run(reqs)
     assert all(r.t_done is not None for r in reqs)
     assert srv.batched_requests / srv.batches > 1.0
+    srv.close()
     alone = serve.replay_alone(sessions, reqs)
     for r, a in zip(reqs, alone):
         assert abs(r.lnl - a) <= REL * abs(a), (r.rid, r.sid, r.kind, r.lnl, a)

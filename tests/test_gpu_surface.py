@@ -105,3 +105,20 @@ def test_batch_poll():
     assert np.array_equal(got, expect)
     with pytest.raises(ValueError):
         nat.batch_poll(ctx, tk)             # collected once
+
+
+def test_tree_destroyed_before_its_likelihoods():
+    # A tree may go before likelihoods created on it, including a clone that
+    # never evaluated (it holds no polarity): they are detached, and
+    # destroying them afterwards frees only their handles.
+    say("tree before liks")
+    prob = synth.make_problem(10, 200, 84)
+    lik = Lik(prob.tree, prob.alignment, ncat=2)
+    synth.configure(lik, prob.model)
+    mate = lik.clone()
+    lik.lnL()
+    lib = nat.lib()
+    lik._dev.close()
+    for L in (mate, lik):
+        assert lib.crux_lik_destroy(L._handle) == 0
+        L._handle = None
