@@ -75,6 +75,32 @@ def available_cores():
 # ---------------------------------------------------------------------------
 # Single large DNA likelihood (the headline).
 
+def tuner_sweep(lik, root):
+    """Traversal time at every launch shape of lik's class (the autotuner's
+    grid), next to the untuned default the other figures are measured at.
+    Leaves the class untuned."""
+    from crux_amd import autotune
+
+    ctx = lik._ctx
+    default = ctx.launch_shape(lik.dim, autotune._ncomp(lik))
+    best, times = autotune.tune(lik, root, reps=10, use_cache=False)
+    tuned = ctx.launch_shape(lik.dim, autotune._ncomp(lik))
+    autotune.set_class_shape(lik, None)
+    # The default is one of the grid's launches: its time is that launch's.
+    by, dflt_ms = {}, None
+    for sh, ms in times.items():
+        autotune.set_class_shape(lik, sh)
+        launch = ctx.launch_shape(lik.dim, autotune._ncomp(lik))
+        by["depth %d chunk %d waves %d" % launch[:3]] = round(ms, 4)
+        if launch == default:
+            dflt_ms = ms
+    autotune.set_class_shape(lik, None)
+    return {"default_launch": {"depth": default[0], "chunk": default[1], "waves": default[2]},
+            "tuned_launch": {"depth": tuned[0], "chunk": tuned[1], "waves": tuned[2]},
+            "default_ms": None if dflt_ms is None else round(dflt_ms, 4),
+            "tuned_ms": round(times[best], 4), "traverse_ms_by_shape": by}
+
+
 def bench_dna(args, out):
     from crux_amd import _native as nat
     from crux_amd import synth
@@ -127,6 +153,11 @@ def bench_dna(args, out):
     achieved = traffic / (trav_ms * 1e-3) / 1e9
     log("dna traversal %.4f ms, P-matrix %.4f ms, gap %.4f ms: %.1f GB/s" %
         (trav_ms, float(np.mean(pm)), float(np.mean(gap)), achieved))
+
+    tune = tuner_sweep(lik, root) if args.stack_depth is None else None
+    if tune:
+        log("dna shapes: default %.4f ms, tuned %.4f ms %s" %
+            (tune["default_ms"], tune["tuned_ms"], tune["tuned_launch"]))
 
     # Plain path: one launch per node update.
     ctx.set_mode(1)
@@ -187,6 +218,7 @@ def bench_dna(args, out):
         "plain_gpu_evals_per_s": round(plain_eps, 2),
         "speedup_vs_plain_gpu": round(full_eps / plain_eps, 2),
         "lnL": lnl0,
+        "autotune": tune,
         "kernel_ms": {"pmat": round(float(np.mean(pm)), 4), "traverse": round(trav_ms, 4),
                       "launch_gap": round(float(np.mean(gap)), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -345,6 +377,9 @@ def bench_aa(args):
     tflops = flops / (trav * 1e-3) / 1e12
     gbs = nbytes / (trav * 1e-3) / 1e9
     log("aa full: %.1f evals/s; traversal %.4f ms; %.1f GB/s; %.1f TFLOP/s" % (eps, trav, gbs, tflops))
+    tune = tuner_sweep(lik, root)
+    log("aa shapes: default %.4f ms, tuned %.4f ms %s" %
+        (tune["default_ms"], tune["tuned_ms"], tune["tuned_launch"]))
     # Opt-in 16-bit inputs to the matrix core (FP32 accumulation; CLVs
     # stay FP32): traversal time and lnL of each, compared with the FP64
     # oracle below.
@@ -376,6 +411,7 @@ def bench_aa(args):
         "matrix_core": {"tflops": round(tflops, 2), "peak_tflops_fp32": FP32_MATRIX_PEAK_TFLOPS,
                         "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4)},
         "input_precisions": precs,
+        "autotune": tune,
     }
 
 
@@ -485,9 +521,8 @@ def main():
     args.serve_rates = [float(x) for x in args.serve_rates.split(",") if x]
     skip = set(x for x in args.skip.split(",") if x)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the likelihood on a single "
-              "device (see DESIGN.md for how the path shards); --gpus %d is not "
-              "supported" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the likelihood runs on a single device; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
     if args.copy_calibration:
         import torch

@@ -19,6 +19,8 @@ CRUX_OK, CRUX_EINVAL, CRUX_ENOMEM, CRUX_ENEGLEN, CRUX_EDECOMP, CRUX_EDEVICE, \
     CRUX_ETAXON, CRUX_ESTATE = range(8)
 CRUX_TUNE_STACK_DEPTH = 1
 CRUX_TUNE_PRECISION = 2
+CRUX_TUNE_CHUNK = 3
+CRUX_TUNE_WAVES = 4
 CRUX_PREC_F32, CRUX_PREC_BF16, CRUX_PREC_F16, CRUX_PREC_F16X3 = range(4)
 
 #: Every entry point the header declares, with (restype, argtypes).
@@ -69,6 +71,12 @@ SIGNATURES = {
     "crux_ctx_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "crux_ctx_get_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "crux_ctx_set_class_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int]),
+    "crux_ctx_get_class_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_int)]),
+    "crux_ctx_launch_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int)]),
     "crux_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
     "crux_ctx_last_traffic": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64),
@@ -203,6 +211,24 @@ class Context:
         v = ctypes.c_int()
         check(lib().crux_ctx_get_tuning(self.handle, int(key), ctypes.byref(v)))
         return v.value
+
+    def set_class_tuning(self, K, C, key, value):
+        """A shape key for the traversal class of K states and C components
+        (-1 clears it: the context's value applies)."""
+        check(lib().crux_ctx_set_class_tuning(self.handle, int(K), int(C), int(key), int(value)))
+
+    def get_class_tuning(self, K, C, key):
+        v = ctypes.c_int()
+        check(lib().crux_ctx_get_class_tuning(self.handle, int(K), int(C), int(key),
+                                              ctypes.byref(v)))
+        return v.value
+
+    def launch_shape(self, K, C):
+        """(stack depth, updates per chunk, waves per SIMD, CM) a batch would
+        launch the class of K states and C components with now."""
+        v = (ctypes.c_int * 4)()
+        check(lib().crux_ctx_launch_shape(self.handle, int(K), int(C), v))
+        return tuple(int(x) for x in v)
 
     def set_timing(self, on):
         check(lib().crux_ctx_set_timing(self.handle, 1 if on else 0))

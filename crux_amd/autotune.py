@@ -1,19 +1,29 @@
-"""Measured choice of the traversal's launch-shape knobs.
+"""Measured choice of the traversal's launch shape.
 
-The traversal kernels keep the results a parent update has not consumed yet
-in a per-lane LDS stack of 2, 4 or 8 slots.  More slots keep more results
-out of HBM; fewer let more blocks share a CU.  Which wins depends on the tree
-shape, the state and component counts and the pattern count, so the tuner
-times full traversals of the caller's own likelihood at each depth (with the
-device's own event timer around the traversal launch) and leaves the fastest
-set on the likelihood's context (the setting is per context: it applies to
-every likelihood evaluated there).
+Three knobs set how a traversal class (state count K, component class CM)
+is launched, none of which changes a result (every choice computes the same
+lnL bit for bit):
+
+* stack depth (2, 4 or 8 LDS slots per lane for results a parent update has
+  not consumed yet): more slots keep more results out of HBM, fewer let more
+  blocks share a CU;
+* 4 states: updates per staged chunk (short, middle, long): the P images and
+  control words of a chunk are staged in LDS together, and the block's waves
+  wait for their stores once per chunk;
+* 20/22 states: waves per SIMD the kernel's registers are held to (1 or 2).
+
+Which shape wins depends on the tree, the state and component counts and the
+pattern count, so the tuner times full traversals of the caller's own
+likelihood at every shape (with the device's own event timer around the
+traversal launch) and keeps the fastest as the setting of that likelihood's
+class on its context; other classes keep theirs.
 
     from crux_amd.autotune import tune
-    best, times = tune(lik)          # times: {depth: traversal ms}
+    best, times = tune(lik)     # best: (depth, chunk, waves) key values,
+                                # times: {key values: traversal ms}
 
-Results are remembered per problem shape (states, component class, pattern
-count, taxa), so tuning a second likelihood of the same shape costs nothing.
+Results are remembered per problem shape (states, components, pattern count,
+taxa), so tuning a second likelihood of the same shape costs nothing.
 """
 
 from __future__ import annotations
@@ -23,14 +33,41 @@ import numpy as np
 from . import _native as nat
 
 DEPTHS = (2, 4, 8)
+CHUNKS = (1, 2, 3)      # short, middle, long (CRUX_TUNE_CHUNK values)
+WAVES = (1, 2)          # CRUX_TUNE_WAVES values
+KEYS = (nat.CRUX_TUNE_STACK_DEPTH, nat.CRUX_TUNE_CHUNK, nat.CRUX_TUNE_WAVES)
 
 _cache: dict = {}
 
 
+def _ncomp(lik):
+    return int(sum(len(m.comps) for m in lik.models))
+
+
 def _shape(lik):
     aln = lik.alignment
-    ncomp = sum(len(m.comps) for m in lik.models)
-    return (lik.dim, int(ncomp), int(aln.nchars), int(aln.ntaxa))
+    return (lik.dim, _ncomp(lik), int(aln.nchars), int(aln.ntaxa))
+
+
+def grid(lik):
+    """The (depth, chunk, waves) key values the tuner tries for lik's class
+    (0: the key does not apply)."""
+    if lik.dim == 4:
+        return [(d, c, 0) for d in DEPTHS for c in CHUNKS]
+    return [(d, 0, w) for d in DEPTHS for w in WAVES]
+
+
+def set_class_shape(lik, shape):
+    """Set (depth, chunk, waves) for lik's class on its context (None: clear,
+    the context's values apply)."""
+    ctx = lik._ctx
+    for key, v in zip(KEYS, shape if shape is not None else (-1, -1, -1)):
+        ctx.set_class_tuning(lik.dim, _ncomp(lik), key, v)
+
+
+def class_shape(lik):
+    ctx = lik._ctx
+    return tuple(ctx.get_class_tuning(lik.dim, _ncomp(lik), key) for key in KEYS)
 
 
 def time_traversal(lik, root=None, reps=8, warmup=2):
@@ -51,24 +88,34 @@ def time_traversal(lik, root=None, reps=8, warmup=2):
     return float(np.median(ts))
 
 
-def tune(lik, root=None, depths=DEPTHS, reps=8, use_cache=True):
-    """Time full traversals of lik at each stack depth, keep the fastest.
+def tune(lik, root=None, shapes=None, reps=8, use_cache=True):
+    """Time full traversals of lik at every launch shape of its class, keep
+    the fastest for the class.
 
-    Returns (best_depth, {depth: median traversal ms}).  The likelihood's
-    cache is flushed by the timing runs (it is rebuilt by the next lnL).
+    Returns (best, {shape: median traversal ms}), shapes as (depth, chunk,
+    waves) key values.  Choices that resolve to the same launch (a depth
+    capped by what fits, waves fixed at depth 8) are timed once.  The
+    likelihood's cache is flushed by the timing runs (it is rebuilt by the
+    next lnL).
     """
     ctx = lik._ctx
     key = _shape(lik)
     if use_cache and key in _cache:
         best, times = _cache[key]
-        ctx.set_tuning(nat.CRUX_TUNE_STACK_DEPTH, best)
+        set_class_shape(lik, best)
         return best, dict(times)
     times = {}
-    for d in depths:
-        ctx.set_tuning(nat.CRUX_TUNE_STACK_DEPTH, int(d))
-        times[int(d)] = time_traversal(lik, root, reps=reps)
-    best = min(times, key=lambda d: (times[d], d))
-    ctx.set_tuning(nat.CRUX_TUNE_STACK_DEPTH, best)
+    seen = set()
+    for sh in shapes if shapes is not None else grid(lik):
+        sh = tuple(int(x) for x in sh)
+        set_class_shape(lik, sh)
+        launch = ctx.launch_shape(lik.dim, _ncomp(lik))
+        if launch in seen:
+            continue
+        seen.add(launch)
+        times[sh] = time_traversal(lik, root, reps=reps)
+    best = min(times, key=lambda s: (times[s], s))
+    set_class_shape(lik, best)
     _cache[key] = (best, dict(times))
     return best, times
 

@@ -15,6 +15,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcrux_amd.so")
 SOURCES = ["engine.hip", "kernels.hip", "model.cpp", "server.cpp"]
+# kernels.hip is compiled once per traversal class (K, CM) plus once for the
+# P-matrix kernel and the dispatch (K 0), so the classes build in parallel.
+KERNEL_PARTS = [(0, 0)] + [(4, cm) for cm in (1, 2, 4, 8, 16)] + \
+    [(k, cm) for k in (20, 22) for cm in (1, 2, 4, 8)]
 ARCH = os.environ.get("CRUX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -27,10 +31,19 @@ def build(verbose=False):
     objs = []
     common = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"),
               "-I", CSRC, "-Wall", "-Wno-unused-result"]
-    procs = []
+    units = []
     for src in SOURCES:
+        if src == "kernels.hip":
+            units += [(src, "_%s.%d_%d.o" % (src, k, cm),
+                       ["-DCRUX_PART_K=%d" % k, "-DCRUX_PART_CM=%d" % cm])
+                      for k, cm in KERNEL_PARTS]
+        else:
+            units.append((src, "_" + src + ".o", []))
+    jobs = max(1, min(len(units), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    running, done = [], []
+    for src, oname, defs in units:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(CSRC, "_" + src + ".o")
+        obj = os.path.join(CSRC, oname)
         if src.endswith(".hip"):
             cmd = [hipcc(), "--offload-arch=" + ARCH, "-x", "hip", "-Xclang",
                    "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -38,17 +51,21 @@ def build(verbose=False):
                    # kernel feeds them to vector instructions at once, and
                    # accumulation registers would cost a move per value.
                    "-mllvm", "-amdgpu-mfma-vgpr-form", "-c",
-                   path, "-o", obj] + common
+                   path, "-o", obj] + defs + common
         else:
             cmd = [hipcc(), "-c", path, "-o", obj] + common
         if verbose:
             print(" ".join(cmd))
-        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE,
-                                       stderr=subprocess.STDOUT), src))
+        while len(running) >= jobs:
+            # Output is read as each unit ends (a full pipe would stall it).
+            p, name = running.pop(0)
+            done.append((p, name, p.communicate()[0]))
+        running.append((subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                         stderr=subprocess.STDOUT), oname))
         objs.append(obj)
+    done += [(p, name, p.communicate()[0]) for p, name in running]
     failed = []
-    for p, src in procs:
-        out, _ = p.communicate()
+    for p, src, out in done:
         text = out.decode(errors="replace")
         lines = [l for l in text.splitlines()
                  if "packed-fp32-ops" not in l and "is not a recognized feature"

@@ -148,6 +148,54 @@ __host__ __device__ inline int stack_depth(int K, int CM, int tuned)
     return K == 4 ? stack_depth4(CM, tuned) : stack_depth20(tuned);
 }
 
+// Updates per staged chunk of the 4-state traversal, three instantiated
+// lengths per (CM, D) (choice 0 short, 1 middle, 2 long).  A chunk ends in a
+// barrier at which the block's waves wait for their stores to drain; a
+// longer chunk has fewer such waits and more LDS per block.  Every length
+// keeps a block within 64 KiB of LDS and its control words within one load
+// per thread (k_trav4: 8 updates, k_trav4h: 16).
+__host__ __device__ constexpr int trav4_chunk(int CM, int D, int choice)
+{
+    return CM == 1    ? (choice == 0 ? 4 : choice == 1 ? 6 : 8)
+           : CM == 2  ? (choice == 0 ? 7 : choice == 1 ? 14 : 16)
+           : CM == 4  ? (D <= 4 ? (choice == 0 ? 6 : choice == 1 ? 11 : 16)
+                                : (choice == 0 ? 4 : choice == 1 ? 8 : 12))
+           : CM == 8  ? (choice == 0 ? 2 : choice == 1 ? 4 : 6)
+                      : (choice == 0 ? 1 : choice == 1 ? 2 : 3);
+}
+// Untuned choice: the longest chunk that lets four blocks (8 waves) share a
+// CU's LDS, the longest of the three for one component.
+__host__ __device__ constexpr int trav4_chunk_default(int CM) { return CM == 1 ? 2 : 1; }
+
+// Launch shape of one traversal class: LDS stack depth, 4-state chunk
+// choice (0-2), waves per SIMD the 20/22-state kernel's registers are held
+// to (1 or 2), input precision (20/22 states).
+struct TravShape {
+    int depth;
+    int chunk;
+    int waves;
+    int prec;
+};
+
+// Traversal classes (K, CM) with a kernel: 4 states at 1-16 components, 20
+// and 22 at 1-8; -1 for none.
+constexpr int kNumClasses = 13;
+__host__ __device__ constexpr int class_index(int K, int CM)
+{
+    const int c = CM == 1 ? 0 : CM == 2 ? 1 : CM == 4 ? 2 : CM == 8 ? 3 : CM == 16 ? 4 : -1;
+    return c < 0 ? -1 : K == 4 ? c : c > 3 ? -1 : K == 20 ? 5 + c : K == 22 ? 9 + c : -1;
+}
+
+// Waves per SIMD the 20/22-state kernel's registers are held to: the tuned
+// value (1 or 2), else two where the registers fit without spilling (up to
+// 256 per lane: 20 states with up to 4 components), so that a chip's worth of
+// waves is resident at once.  At stack depth 8 a block's LDS (over 40 KiB)
+// admits less than one wave per SIMD, so 1.
+__host__ __device__ constexpr int trav20_waves(int KP, int CM, int D, int tuned)
+{
+    return D >= 8 ? 1 : tuned > 0 ? tuned : (KP == 20 && CM <= 4 ? 2 : 1);
+}
+
 struct Entry {
     const Op *ops;
     int32_t nops;

@@ -32,7 +32,7 @@ namespace crux {
 int traverse_cmax(int K, int C);
 // Optional events (null: none) are stamped when the kernel starts and ends.
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, int depth, int prec, hipStream_t stream, hipEvent_t t0,
+                    int blk0, int nblocks, const TravShape &sh, hipStream_t stream, hipEvent_t t0,
                     hipEvent_t t1);
 void launch_pmat(const PJob *jobs, int njobs, size_t shmem, const void *src, void *dst,
                  size_t nbytes, hipStream_t stream, hipEvent_t t0, hipEvent_t t1);
@@ -72,7 +72,12 @@ struct crux_ctx {
     uint64_t used = 0;             // bytes allocated (under amu)
     int mode = 0;
     int depth4 = 0;                // traversal LDS stack depth (0 = default)
+    int chunk4 = 0;                // 4-state chunk choice (0 = default, 1-3)
+    int waves20 = 0;               // 20/22-state waves per SIMD (0 = default)
     int prec = kPrecF32;           // input precision of 20/22-state products
+    // Per-class settings of the three shape keys (-1: the context's value),
+    // indexed by class_index(K, CM).
+    int ctune[kNumClasses][3];
     float *d_zero = nullptr;       // kZeroBytes of zeros (20/22-state kernel)
     bool timing = false;
     double last_ms[4] = {0, 0, 0, 0};
@@ -135,14 +140,25 @@ struct crux_ctx {
     std::mutex amu;
 };
 
-// LDS stack depth of the traversal kernel a batch runs with: the tuned
-// depth, or 4 for the 16-bit-input 20/22-state kernels (instantiated at that
-// depth only).  The planner assigns stack slots for the same depth.
-static int trav_depth(const crux_ctx *ctx, int K, int CM)
+// Launch shape of a traversal class (K, CM) as a batch runs it: the class's
+// tuned values, else the context's, resolved to instantiated ones.  The
+// 16-bit-input 20/22-state kernels are instantiated at stack depth 4 and the
+// default waves only.  The planner assigns stack slots for the same depth.
+static TravShape trav_shape(const crux_ctx *ctx, int K, int CM)
 {
-    if (K != 4 && ctx->prec != kPrecF32)
-        return 4;
-    return stack_depth(K, CM, ctx->depth4);
+    const int ci = class_index(K, CM);
+    auto pick = [&](int k, int dflt) {
+        const int v = ci >= 0 ? ctx->ctune[ci][k] : -1;
+        return v >= 0 ? v : dflt;
+    };
+    TravShape sh;
+    sh.prec = K == 4 ? (int)kPrecF32 : ctx->prec;
+    const bool wide = K == 4 || sh.prec == kPrecF32;
+    sh.depth = wide ? stack_depth(K, CM, pick(0, ctx->depth4)) : 4;
+    const int ch = pick(1, ctx->chunk4);
+    sh.chunk = ch == 0 ? trav4_chunk_default(CM) : ch - 1;
+    sh.waves = trav20_waves(kpad(K), CM, sh.depth, wide ? pick(2, ctx->waves20) : 0);
+    return sh;
 }
 
 static int dev_alloc(crux_ctx *ctx, void **p, size_t bytes)
@@ -977,7 +993,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         // stack; one-launch-per-update mode cannot (LDS does not outlive a
         // launch).
         if (!plain)
-            schedule_stack(L->ops, trav_depth(ctx, K, it.CM));
+            schedule_stack(L->ops, trav_shape(ctx, K, it.CM).depth);
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
         nops += L->ops.size();
@@ -1299,7 +1315,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         if (ln.nblk == 0)
             continue;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
-                            trav_depth(ctx, ln.K, ln.CM), ln.K == 4 ? kPrecF32 : ctx->prec, st,
+                            trav_shape(ctx, ln.K, ln.CM), st,
                             tm && i == first ? S.ev[2] : nullptr,
                             tm && i == last ? S.ev[3] : nullptr) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
@@ -1442,6 +1458,9 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
     crux_ctx *ctx = new crux_ctx;
     ctx->device = device;
     ctx->budget = hbm_budget;
+    for (auto &row : ctx->ctune)
+        for (int &v : row)
+            v = -1;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1525,40 +1544,115 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode)
     return CRUX_OK;
 }
 
-int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value)
+// Range check of a shape key's value (stack depth, chunk choice, waves).
+static int check_shape_value(int key, int value)
 {
-    if (!ctx)
-        return set_error(CRUX_EINVAL, "ctx is NULL");
     switch (key) {
     case CRUX_TUNE_STACK_DEPTH:
         if (value < 0 || value > 64)
             return set_error(CRUX_EINVAL, "stack depth %d", value);
-        ctx->depth4 = value;
         return CRUX_OK;
-    case CRUX_TUNE_PRECISION:
-        if (value < kPrecF32 || value > kPrecF16x3)
-            return set_error(CRUX_EINVAL, "precision %d", value);
-        ctx->prec = value;
+    case CRUX_TUNE_CHUNK:
+        if (value < 0 || value > 3)
+            return set_error(CRUX_EINVAL, "chunk choice %d (0 default, 1-3)", value);
+        return CRUX_OK;
+    case CRUX_TUNE_WAVES:
+        if (value < 0 || value > 2)
+            return set_error(CRUX_EINVAL, "waves per SIMD %d (0 default, 1 or 2)", value);
         return CRUX_OK;
     default:
         return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
     }
 }
 
+static int *ctx_shape_field(crux_ctx *ctx, int key)
+{
+    return key == CRUX_TUNE_STACK_DEPTH ? &ctx->depth4
+           : key == CRUX_TUNE_CHUNK     ? &ctx->chunk4
+                                        : &ctx->waves20;
+}
+
+int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value)
+{
+    if (!ctx)
+        return set_error(CRUX_EINVAL, "ctx is NULL");
+    if (key == CRUX_TUNE_PRECISION) {
+        if (value < kPrecF32 || value > kPrecF16x3)
+            return set_error(CRUX_EINVAL, "precision %d", value);
+        ctx->prec = value;
+        return CRUX_OK;
+    }
+    if (int rc = check_shape_value(key, value))
+        return rc;
+    *ctx_shape_field(ctx, key) = value;
+    return CRUX_OK;
+}
+
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value)
 {
     if (!ctx || !value)
         return set_error(CRUX_EINVAL, "NULL argument");
-    switch (key) {
-    case CRUX_TUNE_STACK_DEPTH:
-        *value = ctx->depth4;
-        return CRUX_OK;
-    case CRUX_TUNE_PRECISION:
+    if (key == CRUX_TUNE_PRECISION) {
         *value = ctx->prec;
         return CRUX_OK;
-    default:
-        return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
     }
+    if (key != CRUX_TUNE_STACK_DEPTH && key != CRUX_TUNE_CHUNK && key != CRUX_TUNE_WAVES)
+        return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
+    *value = *ctx_shape_field(const_cast<crux_ctx *>(ctx), key);
+    return CRUX_OK;
+}
+
+// Class of (K, C) and slot of a shape key in the class table.
+static int class_slot(int K, int C, int key, int *ci, int *ki)
+{
+    const int cm = traverse_cmax(K, C);
+    *ci = cm > 0 ? class_index(K, cm) : -1;
+    if (*ci < 0)
+        return set_error(CRUX_EINVAL, "no traversal class for K=%d C=%d", K, C);
+    *ki = key == CRUX_TUNE_STACK_DEPTH ? 0 : key == CRUX_TUNE_CHUNK ? 1 : key == CRUX_TUNE_WAVES ? 2 : -1;
+    if (*ki < 0)
+        return set_error(CRUX_EINVAL, "tuning key %d is not a launch-shape key", key);
+    return CRUX_OK;
+}
+
+int crux_ctx_set_class_tuning(crux_ctx *ctx, int K, int C, int key, int value)
+{
+    if (!ctx)
+        return set_error(CRUX_EINVAL, "ctx is NULL");
+    int ci, ki;
+    if (int rc = class_slot(K, C, key, &ci, &ki))
+        return rc;
+    if (value != -1)
+        if (int rc = check_shape_value(key, value))
+            return rc;
+    ctx->ctune[ci][ki] = value;
+    return CRUX_OK;
+}
+
+int crux_ctx_get_class_tuning(const crux_ctx *ctx, int K, int C, int key, int *value)
+{
+    if (!ctx || !value)
+        return set_error(CRUX_EINVAL, "NULL argument");
+    int ci, ki;
+    if (int rc = class_slot(K, C, key, &ci, &ki))
+        return rc;
+    *value = ctx->ctune[ci][ki];
+    return CRUX_OK;
+}
+
+int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int *shape4)
+{
+    if (!ctx || !shape4)
+        return set_error(CRUX_EINVAL, "NULL argument");
+    const int cm = traverse_cmax(K, C);
+    if (cm < 0 || class_index(K, cm) < 0)
+        return set_error(CRUX_EINVAL, "no traversal class for K=%d C=%d", K, C);
+    const TravShape sh = trav_shape(ctx, K, cm);
+    shape4[0] = sh.depth;
+    shape4[1] = K == 4 ? trav4_chunk(cm, sh.depth, sh.chunk) : 0;
+    shape4[2] = K == 4 ? 0 : sh.waves;
+    shape4[3] = cm;
+    return CRUX_OK;
 }
 
 int crux_ctx_set_timing(crux_ctx *ctx, int on)

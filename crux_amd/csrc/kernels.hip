@@ -110,6 +110,7 @@ __device__ __forceinline__ void nsum_merge(double &s, double &c, double s2, doub
         c += c2;
 }
 
+#if !defined(CRUX_PART_K) || CRUX_PART_K == 0
 // ---------------------------------------------------------------------------
 // P matrices and tip tables.
 
@@ -229,6 +230,8 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs, int
         J.out[idx] = (float)t;
     }
 }
+
+#endif
 
 // ---------------------------------------------------------------------------
 // Traversal helpers.
@@ -398,20 +401,17 @@ __device__ __forceinline__ float4 mul4(float4 a, float4 b)
 // buffer descriptors (scalar base and range): the address arithmetic stays
 // on the scalar unit, and lanes past the last site fall outside a store's
 // range instead of branching around it.
-template <int CM>
-__host__ __device__ constexpr int trav4_pchunk()
-{
-    return CM <= 4 ? 8 : (CM == 8 ? 4 : 2);
-}
-
-template <int CM, int D>
+//
+// PC: updates per staged chunk (control words of a chunk are one dword per
+// lane).
+template <int CM, int D, int PC>
 __global__ void __launch_bounds__(kTile4)
 k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
     // Two or more components run k_trav4h (the loop below is written for any
     // CM, but only the one-component class is launched).
     static_assert(CM == 1, "k_trav4 serves the one-component class");
-    constexpr int PC = trav4_pchunk<CM>();
+    static_assert(PC * kCtlWords <= kTile4, "a chunk's control words fit one load per lane");
     constexpr int NR = p4_regs(CM);                 // P image registers per update
     constexpr int kChunkRows = PC * NR * 16;        // float4s of P per chunk
     __shared__ float4 stk[D][CM][kTile4];
@@ -731,18 +731,19 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
 // added, and the block's two wave sums are merged in site order.  CLV slots
 // are tiles of 32 sites laid out as the lane halves hold them, so every
 // store writes 1 KiB contiguously; stores are non-temporal.
-template <int CM, int D>
+//
+// PC: updates per staged chunk.  Each chunk ends in a barrier at which both
+// waves wait for their stores to drain, so the untuned length is as long as
+// the LDS allows with four blocks per CU (at most 40 KiB per block: 11
+// updates at four components with up to 4 stack slots, 14 at two).
+template <int CM, int D, int PC>
 __global__ void __launch_bounds__(2 * kTile4)
 k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
     static_assert(CM >= 2, "at least one component per lane half");
+    static_assert(PC * kCtlWords <= 2 * kTile4, "a chunk's control words fit one load per thread");
     constexpr int CL = CM / 2;                    // components per lane
     constexpr int HS = kTile4 / 2;                // sites per wave
-    // Updates per staged chunk: each chunk ends in a barrier at which both
-    // waves wait for their stores to drain, so chunks are as long as the LDS
-    // allows with four blocks per CU (at most 40 KiB per block: 11 updates
-    // at four components with up to 4 stack slots, 14 at two components).
-    constexpr int PC = CM == 4 && D <= 4 ? 11 : CM == 2 ? 14 : trav4_pchunk<CM>();
     constexpr int NR = p4_regs(CM);               // P image registers per update
     constexpr int kChunkRows = PC * NR * 16;      // float4s of P per chunk
     __shared__ float4 stk[2][D][CL][kTile4];
@@ -1252,17 +1253,10 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 // of an update are loaded two updates ahead (one dword per lane, read out
 // with readlane).  A third child (the root update and chained
 // multifurcations) is loaded where it is used.
-// Waves per SIMD the register allocation is held to: two where the registers
-// fit without spilling (up to 256 per lane), so that a chip's worth of waves
-// is resident at once.
-template <int KP, int CM>
-__host__ __device__ constexpr int trav20_waves()
-{
-    return KP == 20 && CM <= 4 ? 2 : 1;
-}
-
-template <int KP, int CM, int D, int PREC>
-__global__ void __launch_bounds__(64, (trav20_waves<KP, CM>()))
+// WV: waves per SIMD the register allocation is held to (1: up to 512
+// registers per lane, 2: up to 256).
+template <int KP, int CM, int D, int PREC, int WV>
+__global__ void __launch_bounds__(64, WV)
 k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
     constexpr int G = sites20(CM);                  // sites per wave
@@ -1645,59 +1639,117 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 // Launches take optional start and stop events (null: none), which the
 // runtime stamps when the kernel itself starts and ends.
 
-template <int CM, int D>
+template <int CM, int D, int PC>
 static void launch_trav4(const Entry *entries, const int32_t *block_entry, int blk0,
                          int nblocks, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
-    if constexpr (D > stack_depth4_max(CM))
-        return;
-    else if constexpr (CM == 1)
-        hipExtLaunchKernelGGL((k_trav4<CM, D>), dim3(nblocks), dim3(kTile4), 0, stream, t0, t1, 0,
-                              entries, block_entry, blk0);
+    if constexpr (CM == 1)
+        hipExtLaunchKernelGGL((k_trav4<CM, D, PC>), dim3(nblocks), dim3(kTile4), 0, stream, t0, t1,
+                              0, entries, block_entry, blk0);
     else
-        hipExtLaunchKernelGGL((k_trav4h<CM, D>), dim3(nblocks), dim3(2 * kTile4), 0, stream, t0,
-                              t1, 0, entries, block_entry, blk0);
+        hipExtLaunchKernelGGL((k_trav4h<CM, D, PC>), dim3(nblocks), dim3(2 * kTile4), 0, stream,
+                              t0, t1, 0, entries, block_entry, blk0);
 }
 
-template <int KP, int CM, int D, int PREC>
+template <int CM, int D>
+static void launch_trav4_chunk(const Entry *entries, const int32_t *block_entry, int blk0,
+                               int nblocks, int chunk, hipStream_t stream, hipEvent_t t0,
+                               hipEvent_t t1)
+{
+    if constexpr (D > stack_depth4_max(CM)) {
+        return;
+    } else {
+        if (chunk == 0)
+            launch_trav4<CM, D, trav4_chunk(CM, D, 0)>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (chunk == 2)
+            launch_trav4<CM, D, trav4_chunk(CM, D, 2)>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else
+            launch_trav4<CM, D, trav4_chunk(CM, D, 1)>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+    }
+}
+
+template <int KP, int CM, int D, int PREC, int WV>
 static void launch_trav20(const Entry *entries, const int32_t *block_entry, int blk0,
                           int nblocks, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
-    hipExtLaunchKernelGGL((k_trav20<KP, CM, D, PREC>), dim3(nblocks), dim3(64), 0, stream, t0, t1, 0,
-                          entries, block_entry, blk0);
+    hipExtLaunchKernelGGL((k_trav20<KP, CM, D, PREC, WV>), dim3(nblocks), dim3(64), 0, stream, t0,
+                          t1, 0, entries, block_entry, blk0);
 }
 
+template <int KP, int CM, int D>
+static void launch_trav20_waves(const Entry *entries, const int32_t *block_entry, int blk0,
+                                int nblocks, int waves, hipStream_t stream, hipEvent_t t0,
+                                hipEvent_t t1)
+{
+    if constexpr (D < 8) {
+        if (waves == 2) {
+            launch_trav20<KP, CM, D, kPrecF32, 2>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+            return;
+        }
+    }
+    launch_trav20<KP, CM, D, kPrecF32, 1>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+}
+
+// The traversal launch of one (K, CM) class at a launch shape (depth, chunk
+// and waves already resolved to instantiated values).
 template <int K, int CM>
-static void launch_traverse(const Entry *entries, const int32_t *block_entry, int blk0,
-                            int nblocks, int depth, int prec, hipStream_t stream, hipEvent_t t0,
-                            hipEvent_t t1)
+void launch_class(const Entry *entries, const int32_t *block_entry, int blk0, int nblocks,
+                  const TravShape &sh, hipStream_t stream, hipEvent_t t0, hipEvent_t t1)
 {
     // Stack depth: more slots keep more results in LDS, fewer let more
     // blocks share a CU.
     if constexpr (K == 4) {
-        if (depth <= 2)
-            launch_trav4<CM, 2>(entries, block_entry, blk0, nblocks, stream, t0, t1);
-        else if (depth <= 4)
-            launch_trav4<CM, 4>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        if (sh.depth <= 2)
+            launch_trav4_chunk<CM, 2>(entries, block_entry, blk0, nblocks, sh.chunk, stream, t0, t1);
+        else if (sh.depth <= 4)
+            launch_trav4_chunk<CM, 4>(entries, block_entry, blk0, nblocks, sh.chunk, stream, t0, t1);
         else
-            launch_trav4<CM, 8>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+            launch_trav4_chunk<CM, 8>(entries, block_entry, blk0, nblocks, sh.chunk, stream, t0, t1);
     } else {
         constexpr int KP = kpad(K);
-        // 16-bit inputs are instantiated at the default stack depth only.
-        if (prec == kPrecBF16)
-            launch_trav20<KP, CM, 4, kPrecBF16>(entries, block_entry, blk0, nblocks, stream, t0, t1);
-        else if (prec == kPrecF16)
-            launch_trav20<KP, CM, 4, kPrecF16>(entries, block_entry, blk0, nblocks, stream, t0, t1);
-        else if (prec == kPrecF16x3)
-            launch_trav20<KP, CM, 4, kPrecF16x3>(entries, block_entry, blk0, nblocks, stream, t0, t1);
-        else if (depth <= 2)
-            launch_trav20<KP, CM, 2, kPrecF32>(entries, block_entry, blk0, nblocks, stream, t0, t1);
-        else if (depth <= 4)
-            launch_trav20<KP, CM, 4, kPrecF32>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        constexpr int WD = trav20_waves(KP, CM, 4, 0);
+        // 16-bit inputs are instantiated at the default stack depth and
+        // waves only.
+        if (sh.prec == kPrecBF16)
+            launch_trav20<KP, CM, 4, kPrecBF16, WD>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (sh.prec == kPrecF16)
+            launch_trav20<KP, CM, 4, kPrecF16, WD>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (sh.prec == kPrecF16x3)
+            launch_trav20<KP, CM, 4, kPrecF16x3, WD>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+        else if (sh.depth <= 2)
+            launch_trav20_waves<KP, CM, 2>(entries, block_entry, blk0, nblocks, sh.waves, stream, t0, t1);
+        else if (sh.depth <= 4)
+            launch_trav20_waves<KP, CM, 4>(entries, block_entry, blk0, nblocks, sh.waves, stream, t0, t1);
         else
-            launch_trav20<KP, CM, 8, kPrecF32>(entries, block_entry, blk0, nblocks, stream, t0, t1);
+            launch_trav20_waves<KP, CM, 8>(entries, block_entry, blk0, nblocks, sh.waves, stream, t0, t1);
     }
 }
+
+// The instantiated classes.  Built in parts (CRUX_PART_K / CRUX_PART_CM name
+// the one class a compilation of this file instantiates, CRUX_PART_K 0 the
+// P-matrix kernel and the dispatch), so that the classes compile in
+// parallel; without CRUX_PART_K the file holds everything.
+#define CRUX_CLASSES(X)                                               \
+    X(4, 1) X(4, 2) X(4, 4) X(4, 8) X(4, 16)                          \
+    X(20, 1) X(20, 2) X(20, 4) X(20, 8)                               \
+    X(22, 1) X(22, 2) X(22, 4) X(22, 8)
+#define CRUX_LAUNCH_ARGS                                                                         \
+    const Entry *, const int32_t *, int, int, const TravShape &, hipStream_t, hipEvent_t,       \
+        hipEvent_t
+
+#if !defined(CRUX_PART_K)
+#define CRUX_INSTANTIATE(KK, CC) template void launch_class<KK, CC>(CRUX_LAUNCH_ARGS);
+CRUX_CLASSES(CRUX_INSTANTIATE)
+#undef CRUX_INSTANTIATE
+#elif CRUX_PART_K != 0
+template void launch_class<CRUX_PART_K, CRUX_PART_CM>(CRUX_LAUNCH_ARGS);
+#else
+#define CRUX_EXTERN(KK, CC) extern template void launch_class<KK, CC>(CRUX_LAUNCH_ARGS);
+CRUX_CLASSES(CRUX_EXTERN)
+#undef CRUX_EXTERN
+#endif
+
+#if !defined(CRUX_PART_K) || CRUX_PART_K == 0
 
 // Component-count class of an evaluation: the smallest instantiated CM >= C.
 int traverse_cmax(int K, int C)
@@ -1718,17 +1770,15 @@ int traverse_cmax(int K, int C)
 }
 
 int launch_traverse(int K, int CM, const Entry *entries, const int32_t *block_entry,
-                    int blk0, int nblocks, int depth, int prec, hipStream_t stream, hipEvent_t t0,
+                    int blk0, int nblocks, const TravShape &sh, hipStream_t stream, hipEvent_t t0,
                     hipEvent_t t1)
 {
-#define CRUX_CASE(KK, CC)                                                                       \
-    if (K == KK && CM == CC) {                                                                  \
-        launch_traverse<KK, CC>(entries, block_entry, blk0, nblocks, depth, prec, stream, t0, t1); \
-        return 0;                                                                               \
+#define CRUX_CASE(KK, CC)                                                                    \
+    if (K == KK && CM == CC) {                                                               \
+        launch_class<KK, CC>(entries, block_entry, blk0, nblocks, sh, stream, t0, t1);       \
+        return 0;                                                                            \
     }
-    CRUX_CASE(4, 1) CRUX_CASE(4, 2) CRUX_CASE(4, 4) CRUX_CASE(4, 8) CRUX_CASE(4, 16)
-    CRUX_CASE(20, 1) CRUX_CASE(20, 2) CRUX_CASE(20, 4) CRUX_CASE(20, 8)
-    CRUX_CASE(22, 1) CRUX_CASE(22, 2) CRUX_CASE(22, 4) CRUX_CASE(22, 8)
+    CRUX_CLASSES(CRUX_CASE)
 #undef CRUX_CASE
     return -1;
 }
@@ -1743,5 +1793,6 @@ void launch_pmat(const PJob *jobs, int njobs, size_t shmem, const void *src, voi
                           jobs, njobs, static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
 }
 
+#endif
 
 }  // namespace crux

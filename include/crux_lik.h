@@ -244,21 +244,46 @@ int crux_server_stats(crux_server *srv, uint64_t *stats4);
 int crux_ctx_set_mode(crux_ctx *ctx, int mode);
 
 /* Launch-shape tuning of the context (set by the autotuner,
- * crux_amd/autotune.py):
+ * crux_amd/autotune.py).  No counterpart in the reference, whose stripe
+ * width is fixed (CxLik.h:176-273).  None of the shape keys changes a
+ * result: every choice computes the same lnL bit for bit.
  *   CRUX_TUNE_STACK_DEPTH: LDS stack slots per lane of the traversal
  *   kernels (2, 4 or 8, capped by what fits; 0 = default: 8 for 4 states
  *   with up to 2 components, else 4).  Fewer slots let more blocks share a
  *   CU; more slots keep more pending results out of HBM.
+ *   CRUX_TUNE_CHUNK: updates per staged chunk of the 4-state kernels, whose
+ *   P images and control words are staged in LDS a chunk at a time and whose
+ *   waves wait for their stores at each chunk's end: 0 = default, 1 short,
+ *   2 middle, 3 long (per component class 1/2/4/8/16: 4-6-8, 7-14-16,
+ *   6-11-16 (4-8-12 at stack depth 8), 2-4-6, 1-2-3 updates; the default is
+ *   the middle length, the long one for one component).
+ *   CRUX_TUNE_WAVES: waves per SIMD the 20/22-state kernel's registers are
+ *   held to, 1 (up to 512 registers per lane) or 2 (up to 256); 0 = default
+ *   (2 for 20 states with up to 4 components, else 1).
  *   CRUX_TUNE_PRECISION: input precision of the 20/22-state P * CLV
  *   products on the matrix core, accumulated in FP32 (CLVs are stored in
  *   FP32 either way): CRUX_PREC_F32 (default), CRUX_PREC_BF16, CRUX_PREC_F16,
  *   or CRUX_PREC_F16X3 (float16 high and low parts, three products).  The
- *   16-bit forms run at stack depth 4.  Likelihoods evaluated at another
- *   precision recompute their CLVs. */
+ *   16-bit forms run at stack depth 4 and the default waves.  Likelihoods
+ *   evaluated at another precision recompute their CLVs.
+ * Settings apply to batches submitted after the call; do not change them
+ * while another thread submits on the context. */
 enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
-enum { CRUX_TUNE_STACK_DEPTH = 1, CRUX_TUNE_PRECISION = 2 };
+enum { CRUX_TUNE_STACK_DEPTH = 1, CRUX_TUNE_PRECISION = 2, CRUX_TUNE_CHUNK = 3, CRUX_TUNE_WAVES = 4 };
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
+
+/* The same shape keys (not precision) for one traversal class: K states and
+ * C components (the class of the smallest instantiated component count >=
+ * C).  value -1 clears the class's setting (the context's value applies).
+ * A batch mixing classes launches each with its own shape. */
+int crux_ctx_set_class_tuning(crux_ctx *ctx, int K, int C, int key, int value);
+int crux_ctx_get_class_tuning(const crux_ctx *ctx, int K, int C, int key, int *value);
+
+/* The launch shape a batch would use now for K states and C components:
+ * shape4 = {stack depth, updates per chunk (4 states; else 0), waves per
+ * SIMD (20/22 states; else 0), component class CM}. */
+int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int *shape4);
 
 /* Timing of the last evaluation on the device stream, in milliseconds, from
  * events the runtime stamps when each kernel starts and ends: [0] P-matrix
