@@ -82,15 +82,15 @@ def tuner_sweep(lik, root):
     from crux_amd import autotune
 
     ctx = lik._ctx
-    default = ctx.launch_shape(lik.dim, autotune._ncomp(lik))
+    default = autotune.launch_of(lik)
     best, times = autotune.tune(lik, root, reps=10, use_cache=False)
-    tuned = ctx.launch_shape(lik.dim, autotune._ncomp(lik))
+    tuned = autotune.launch_of(lik)
     autotune.set_class_shape(lik, None)
     # The default is one of the grid's launches: its time is that launch's.
     by, dflt_ms = {}, None
     for sh, ms in times.items():
         autotune.set_class_shape(lik, sh)
-        launch = ctx.launch_shape(lik.dim, autotune._ncomp(lik))
+        launch = autotune.launch_of(lik)
         by["depth %d chunk %d waves %d" % launch[:3]] = round(ms, 4)
         if launch == default:
             dflt_ms = ms

@@ -75,7 +75,7 @@ SIGNATURES = {
                                                  ctypes.c_int]),
     "crux_ctx_get_class_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_int)]),
-    "crux_ctx_launch_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+    "crux_ctx_launch_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_int)]),
     "crux_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
@@ -223,11 +223,12 @@ class Context:
                                               ctypes.byref(v)))
         return v.value
 
-    def launch_shape(self, K, C):
-        """(stack depth, updates per chunk, waves per SIMD, CM) a batch would
-        launch the class of K states and C components with now."""
+    def launch_shape(self, K, C, S):
+        """(stack depth, updates per chunk, waves per SIMD, CM) a batch of one
+        evaluation of S patterns would launch the class of K states and C
+        components with now."""
         v = (ctypes.c_int * 4)()
-        check(lib().crux_ctx_launch_shape(self.handle, int(K), int(C), v))
+        check(lib().crux_ctx_launch_shape(self.handle, int(K), int(C), int(S), v))
         return tuple(int(x) for x in v)
 
     def set_timing(self, on):

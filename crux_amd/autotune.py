@@ -44,6 +44,12 @@ def _ncomp(lik):
     return int(sum(len(m.comps) for m in lik.models))
 
 
+def launch_of(lik):
+    """(stack depth, updates per chunk, waves per SIMD, CM) lik's evaluation
+    launches with now."""
+    return lik._ctx.launch_shape(lik.dim, _ncomp(lik), int(lik.alignment.nchars))
+
+
 def _shape(lik):
     aln = lik.alignment
     return (lik.dim, _ncomp(lik), int(aln.nchars), int(aln.ntaxa))
@@ -98,7 +104,6 @@ def tune(lik, root=None, shapes=None, reps=8, use_cache=True):
     likelihood's cache is flushed by the timing runs (it is rebuilt by the
     next lnL).
     """
-    ctx = lik._ctx
     key = _shape(lik)
     if use_cache and key in _cache:
         best, times = _cache[key]
@@ -109,7 +114,7 @@ def tune(lik, root=None, shapes=None, reps=8, use_cache=True):
     for sh in shapes if shapes is not None else grid(lik):
         sh = tuple(int(x) for x in sh)
         set_class_shape(lik, sh)
-        launch = ctx.launch_shape(lik.dim, _ncomp(lik))
+        launch = launch_of(lik)
         if launch in seen:
             continue
         seen.add(launch)

@@ -122,13 +122,11 @@ __host__ __device__ constexpr int img_words(int KP, int CM, int prec)
 // class CM (keeps the stack at or below 32 KiB per block).
 __host__ __device__ constexpr int stack_depth4_max(int CM) { return CM <= 4 ? 8 : (CM == 8 ? 4 : 2); }
 
-// Stack depth actually used: the tuned value (0 = largest), rounded down to
-// an instantiated depth (2, 4 or 8).
-__host__ __device__ inline int stack_depth4(int CM, int tuned)
+// Stack depth actually used: the tuned value, capped by what fits and
+// rounded down to an instantiated depth (2, 4 or 8); untuned: the default.
+__host__ __device__ inline int stack_depth4(int CM, int tuned, int dflt)
 {
-    // Untuned: 8 slots while a block's LDS stays small enough for every block
-    // of a 60k-site evaluation to be resident at once, else 4.
-    int d = tuned > 0 ? tuned : (CM <= 2 ? 8 : 4);
+    int d = tuned > 0 ? tuned : dflt;
     if (d > stack_depth4_max(CM))
         d = stack_depth4_max(CM);
     return d >= 8 ? 8 : (d >= 4 ? 4 : 2);
@@ -143,10 +141,6 @@ __host__ __device__ inline int stack_depth20(int tuned)
     return d >= 8 ? 8 : (d >= 4 ? 4 : 2);
 }
 
-__host__ __device__ inline int stack_depth(int K, int CM, int tuned)
-{
-    return K == 4 ? stack_depth4(CM, tuned) : stack_depth20(tuned);
-}
 
 // Updates per staged chunk of the 4-state traversal, three instantiated
 // lengths per (CM, D) (choice 0 short, 1 middle, 2 long).  A chunk ends in a
@@ -163,9 +157,40 @@ __host__ __device__ constexpr int trav4_chunk(int CM, int D, int choice)
            : CM == 8  ? (choice == 0 ? 2 : choice == 1 ? 4 : 6)
                       : (choice == 0 ? 1 : choice == 1 ? 2 : 3);
 }
-// Untuned choice: the longest chunk that lets four blocks (8 waves) share a
-// CU's LDS, the longest of the three for one component.
-__host__ __device__ constexpr int trav4_chunk_default(int CM) { return CM == 1 ? 2 : 1; }
+// Untuned 4-state launch shape of a class launch of `tiles` blocks on `ncu`
+// CUs.  A block walks the whole plan for its sites, so a launch with more
+// blocks than fit on the chip at once runs a second round of whole block
+// lifetimes; the shapes below were the fastest (or within a few percent of
+// it) on MI355X (256 CUs) for 128 taxa and 256-4096 tiles per launch:
+//   CM 1:  depth 8, 8 updates per chunk; above 8 tiles per CU depth 4, 6
+//   CM 2:  depth 4, 16; above 5 tiles per CU 7
+//   CM 4:  depth 4, 11; up to 2 tiles per CU depth 8, 12 (two such blocks
+//          fit on a CU); at 4-5 tiles per CU depth 2, 6 (six blocks per CU
+//          fit where four do at depth 4)
+//   CM 8:  depth 4, 4
+//   CM 16: depth 2, 3
+// (chunk lengths as choices of trav4_chunk).
+__host__ __device__ inline void default_shape4(int CM, long tiles, int ncu, int *depth, int *chunk)
+{
+    const double t = (double)tiles / (ncu > 0 ? ncu : 256);
+    if (CM == 1) {
+        *depth = t <= 8 ? 8 : 4;
+        *chunk = t <= 8 ? 2 : 1;
+    } else if (CM == 2) {
+        *depth = 4;
+        *chunk = t <= 5 ? 2 : 0;
+    } else if (CM == 4) {
+        const bool few = t <= 2, mid = t > 4 && t <= 5;
+        *depth = few ? 8 : mid ? 2 : 4;
+        *chunk = few ? 2 : mid ? 0 : 1;
+    } else if (CM == 8) {
+        *depth = 4;
+        *chunk = 1;
+    } else {
+        *depth = 2;
+        *chunk = 2;
+    }
+}
 
 // Launch shape of one traversal class: LDS stack depth, 4-state chunk
 // choice (0-2), waves per SIMD the 20/22-state kernel's registers are held

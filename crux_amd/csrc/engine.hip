@@ -71,6 +71,7 @@ struct crux_ctx {
     uint64_t budget = 0;
     uint64_t used = 0;             // bytes allocated (under amu)
     int mode = 0;
+    int ncu = 256;                 // compute units of the device
     int depth4 = 0;                // traversal LDS stack depth (0 = default)
     int chunk4 = 0;                // 4-state chunk choice (0 = default, 1-3)
     int waves20 = 0;               // 20/22-state waves per SIMD (0 = default)
@@ -140,11 +141,13 @@ struct crux_ctx {
     std::mutex amu;
 };
 
-// Launch shape of a traversal class (K, CM) as a batch runs it: the class's
-// tuned values, else the context's, resolved to instantiated ones.  The
-// 16-bit-input 20/22-state kernels are instantiated at stack depth 4 and the
-// default waves only.  The planner assigns stack slots for the same depth.
-static TravShape trav_shape(const crux_ctx *ctx, int K, int CM)
+// Launch shape of a traversal class (K, CM) as a batch runs it, `tiles`
+// blocks in the class's launch: the class's tuned values, else the
+// context's, resolved to instantiated ones; untuned keys take the default
+// for the launch's tiles per CU.  The 16-bit-input 20/22-state kernels are
+// instantiated at stack depth 4 and the default waves only.  The planner
+// assigns stack slots for the same depth.
+static TravShape trav_shape(const crux_ctx *ctx, int K, int CM, long tiles)
 {
     const int ci = class_index(K, CM);
     auto pick = [&](int k, int dflt) {
@@ -154,9 +157,16 @@ static TravShape trav_shape(const crux_ctx *ctx, int K, int CM)
     TravShape sh;
     sh.prec = K == 4 ? (int)kPrecF32 : ctx->prec;
     const bool wide = K == 4 || sh.prec == kPrecF32;
-    sh.depth = wide ? stack_depth(K, CM, pick(0, ctx->depth4)) : 4;
-    const int ch = pick(1, ctx->chunk4);
-    sh.chunk = ch == 0 ? trav4_chunk_default(CM) : ch - 1;
+    const int dt = pick(0, ctx->depth4), ct = pick(1, ctx->chunk4);
+    if (K == 4) {
+        int dd, dc;
+        default_shape4(CM, tiles, ctx->ncu, &dd, &dc);
+        sh.depth = stack_depth4(CM, dt, dd);
+        sh.chunk = ct > 0 ? ct - 1 : dc;
+    } else {
+        sh.depth = wide ? stack_depth20(dt) : 4;
+        sh.chunk = 0;
+    }
     sh.waves = trav20_waves(kpad(K), CM, sh.depth, wide ? pick(2, ctx->waves20) : 0);
     return sh;
 }
@@ -981,6 +991,16 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     });
 
     const bool plain = ctx->mode == 1;
+    // Launch shape of each class, from its blocks in this batch.
+    long cls_tiles[kNumClasses] = {};
+    for (Item &it : items)
+        cls_tiles[class_index(it.L->tree->K, it.CM)] +=
+            (it.L->tree->S + tile_sites(it.L->tree->K, it.CM) - 1) / tile_sites(it.L->tree->K, it.CM);
+    TravShape cls_shape[kNumClasses];
+    for (Item &it : items) {
+        const int ci = class_index(it.L->tree->K, it.CM);
+        cls_shape[ci] = trav_shape(ctx, it.L->tree->K, it.CM, cls_tiles[ci]);
+    }
     // Sizes.
     size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0, ntips_total = 0;
     size_t ctl_words = 0;
@@ -993,7 +1013,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         // stack; one-launch-per-update mode cannot (LDS does not outlive a
         // launch).
         if (!plain)
-            schedule_stack(L->ops, trav_shape(ctx, K, it.CM).depth);
+            schedule_stack(L->ops, cls_shape[class_index(K, it.CM)].depth);
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
         nops += L->ops.size();
@@ -1315,7 +1335,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         if (ln.nblk == 0)
             continue;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
-                            trav_shape(ctx, ln.K, ln.CM), st,
+                            cls_shape[class_index(ln.K, ln.CM)], st,
                             tm && i == first ? S.ev[2] : nullptr,
                             tm && i == last ? S.ev[3] : nullptr) != 0)
             return set_error(CRUX_EINVAL, "no traversal kernel for K=%d", ln.K);
@@ -1462,6 +1482,8 @@ int crux_ctx_create(int device, uint64_t hbm_budget, crux_ctx **out)
         for (int &v : row)
             v = -1;
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int k = 0; k < 3 && e == hipSuccess; k++) {
@@ -1640,14 +1662,16 @@ int crux_ctx_get_class_tuning(const crux_ctx *ctx, int K, int C, int key, int *v
     return CRUX_OK;
 }
 
-int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int *shape4)
+int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int S, int *shape4)
 {
     if (!ctx || !shape4)
         return set_error(CRUX_EINVAL, "NULL argument");
     const int cm = traverse_cmax(K, C);
     if (cm < 0 || class_index(K, cm) < 0)
         return set_error(CRUX_EINVAL, "no traversal class for K=%d C=%d", K, C);
-    const TravShape sh = trav_shape(ctx, K, cm);
+    if (S < 1)
+        return set_error(CRUX_EINVAL, "pattern count %d", S);
+    const TravShape sh = trav_shape(ctx, K, cm, (S + tile_sites(K, cm) - 1) / tile_sites(K, cm));
     shape4[0] = sh.depth;
     shape4[1] = K == 4 ? trav4_chunk(cm, sh.depth, sh.chunk) : 0;
     shape4[2] = K == 4 ? 0 : sh.waves;

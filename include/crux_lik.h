@@ -248,15 +248,18 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  * width is fixed (CxLik.h:176-273).  None of the shape keys changes a
  * result: every choice computes the same lnL bit for bit.
  *   CRUX_TUNE_STACK_DEPTH: LDS stack slots per lane of the traversal
- *   kernels (2, 4 or 8, capped by what fits; 0 = default: 8 for 4 states
- *   with up to 2 components, else 4).  Fewer slots let more blocks share a
+ *   kernels (2, 4 or 8, capped by what fits; 0 = default: 4 for 20/22
+ *   states, for 4 states see below).  Fewer slots let more blocks share a
  *   CU; more slots keep more pending results out of HBM.
  *   CRUX_TUNE_CHUNK: updates per staged chunk of the 4-state kernels, whose
  *   P images and control words are staged in LDS a chunk at a time and whose
  *   waves wait for their stores at each chunk's end: 0 = default, 1 short,
  *   2 middle, 3 long (per component class 1/2/4/8/16: 4-6-8, 7-14-16,
- *   6-11-16 (4-8-12 at stack depth 8), 2-4-6, 1-2-3 updates; the default is
- *   the middle length, the long one for one component).
+ *   6-11-16 (4-8-12 at stack depth 8), 2-4-6, 1-2-3 updates).
+ *   Untuned 4-state depth and chunk follow the launch's blocks per CU: a
+ *   block walks the whole plan for its 64 sites, so a launch with more
+ *   blocks than fit on the chip at once runs a second round of whole block
+ *   lifetimes, and a shape with less LDS per block may fit them all.
  *   CRUX_TUNE_WAVES: waves per SIMD the 20/22-state kernel's registers are
  *   held to, 1 (up to 512 registers per lane) or 2 (up to 256); 0 = default
  *   (2 for 20 states with up to 4 components, else 1).
@@ -280,10 +283,13 @@ int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
 int crux_ctx_set_class_tuning(crux_ctx *ctx, int K, int C, int key, int value);
 int crux_ctx_get_class_tuning(const crux_ctx *ctx, int K, int C, int key, int *value);
 
-/* The launch shape a batch would use now for K states and C components:
- * shape4 = {stack depth, updates per chunk (4 states; else 0), waves per
- * SIMD (20/22 states; else 0), component class CM}. */
-int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int *shape4);
+/* The launch shape a batch of one evaluation of S patterns (S after
+ * compaction) would use now for K states and C components: shape4 = {stack
+ * depth, updates per chunk (4 states; else 0), waves per SIMD (20/22 states;
+ * else 0), component class CM}.  Untuned 4-state keys depend on the
+ * launch's blocks per CU (a batch's class launch covers every evaluation of
+ * the class in it). */
+int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int S, int *shape4);
 
 /* Timing of the last evaluation on the device stream, in milliseconds, from
  * events the runtime stamps when each kernel starts and ends: [0] P-matrix

@@ -38,7 +38,7 @@ def test_every_shape_is_bit_identical(kind, ncat):
         launches, parts = set(), []
         for sh in autotune.grid(lik):
             autotune.set_class_shape(lik, sh)
-            launch = ctx.launch_shape(K, ncat)
+            launch = autotune.launch_of(lik)
             launches.add(launch)
             lik.flush()
             got = lik.lnL()
@@ -57,20 +57,21 @@ def test_every_shape_is_bit_identical(kind, ncat):
 def test_class_settings_override_context():
     _, dna = _lik("dna", 4)
     ctx = dna._ctx
+    S = 1500
     try:
-        base = ctx.launch_shape(4, 4)
+        base = ctx.launch_shape(4, 4, S)
         ctx.set_tuning(nat.CRUX_TUNE_STACK_DEPTH, 2)
-        assert ctx.launch_shape(4, 4)[0] == 2
+        assert ctx.launch_shape(4, 4, S)[0] == 2
         ctx.set_class_tuning(4, 3, nat.CRUX_TUNE_STACK_DEPTH, 8)   # C 3 -> class CM 4
         assert ctx.get_class_tuning(4, 4, nat.CRUX_TUNE_STACK_DEPTH) == 8
-        assert ctx.launch_shape(4, 4)[0] == 8
-        assert ctx.launch_shape(4, 2)[0] == 2      # other class: the context's
+        assert ctx.launch_shape(4, 4, S)[0] == 8
+        assert ctx.launch_shape(4, 2, S)[0] == 2      # other class: the context's
         ctx.set_class_tuning(4, 4, nat.CRUX_TUNE_CHUNK, 1)
-        assert ctx.launch_shape(4, 4)[1] == 4      # short chunk at depth 8, CM 4
+        assert ctx.launch_shape(4, 4, S)[1] == 4      # short chunk at depth 8, CM 4
         ctx.set_class_tuning(20, 4, nat.CRUX_TUNE_WAVES, 1)
-        assert ctx.launch_shape(20, 4)[2] == 1
+        assert ctx.launch_shape(20, 4, S)[2] == 1
         ctx.set_class_tuning(20, 4, nat.CRUX_TUNE_STACK_DEPTH, 8)
-        assert ctx.launch_shape(20, 4)[2] == 1
+        assert ctx.launch_shape(20, 4, S)[2] == 1
         with pytest.raises(ValueError):
             ctx.set_class_tuning(4, 4, nat.CRUX_TUNE_CHUNK, 4)
         with pytest.raises(ValueError):
@@ -84,7 +85,7 @@ def test_class_settings_override_context():
         for K, C in ((4, 4), (20, 4)):
             for key in KEYS:
                 ctx.set_class_tuning(K, C, key, -1)
-    assert ctx.launch_shape(4, 4) == base
+    assert ctx.launch_shape(4, 4, S) == base
 
 
 @pytest.mark.parametrize("kind", ["dna", "aa"])
@@ -95,14 +96,14 @@ def test_tuner_keeps_a_timed_shape_and_caches(kind):
     try:
         ref = lik.lnL()
         other_class = (20, 4) if kind == "dna" else (4, 4)
-        before = ctx.launch_shape(*other_class)
+        before = ctx.launch_shape(*other_class, 1500)
         autotune.clear()
         best, times = autotune.tune(lik, reps=2)
         assert best in times and set(times) <= set(autotune.grid(lik))
         assert len(times) >= 2
         assert all(t > 0.0 for t in times.values())
         assert autotune.class_shape(lik) == best
-        assert ctx.launch_shape(*other_class) == before
+        assert ctx.launch_shape(*other_class, 1500) == before
         assert lik.lnL() == ref
         # Same shape: answered from the cache.
         other = Lik(prob.fresh_tree(), prob.alignment, ncat=4)
