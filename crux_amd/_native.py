@@ -23,6 +23,10 @@ CRUX_TUNE_CHUNK = 3
 CRUX_TUNE_WAVES = 4
 CRUX_PREC_F32, CRUX_PREC_BF16, CRUX_PREC_F16, CRUX_PREC_F16X3 = range(4)
 
+# Mixture components (over all models, +I counted) a likelihood may have:
+# the largest component class of the traversal kernels per state count.
+MAX_COMPONENTS = {4: 16, 20: 8, 22: 8}
+
 #: Every entry point the header declares, with (restype, argtypes).
 _vp = ctypes.c_void_p
 _i32p = ctypes.POINTER(ctypes.c_int32)

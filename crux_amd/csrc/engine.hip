@@ -380,6 +380,20 @@ static void arena_put(crux_tree *t, int pol, int32_t slot)
         t->arena[pol].free_slots.push_back(slot);
 }
 
+// A ring detached from the tree when its topology is sent: its cached CLVs
+// (both polarities) are dead, since a ring that comes back is re-keyed
+// anyway, so the slots return to the free lists.  Moves that detach and
+// re-attach an edge between two evaluations keep its ring attached at every
+// sync and its caches.
+static void release_ring(crux_tree *t, int32_t r)
+{
+    for (int p = 0; p < 2; p++)
+        if (r < (int32_t)t->cache[p].size()) {
+            arena_put(t, p, t->cache[p][r].slot);
+            t->cache[p][r] = RingCache{};
+        }
+}
+
 static int arena_layout(crux_tree *t, int pol, int C)
 {
     Arena &a = t->arena[pol];
@@ -1864,6 +1878,9 @@ int crux_tree_set_topology(crux_tree *t, int nnodes, int nedges, const int32_t *
             t->cache[p].resize(nr);
         }
     }
+    for (int r = 0; r < nr; r++)
+        if (ring_node[r] < 0)
+            release_ring(t, r);
     t->has_topology = true;
     return CRUX_OK;
 }
@@ -1944,6 +1961,9 @@ int crux_tree_update_rings(crux_tree *t, int nrings, const int32_t *rings,
     t->node_ring.swap(nring);
     for (auto &d : deg)
         t->degree[d.first] = d.second;
+    for (int i = 0; i < nrings; i++)
+        if (t->ring_node[rings[i]] < 0)
+            release_ring(t, rings[i]);
     return CRUX_OK;
 }
 

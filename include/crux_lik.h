@@ -101,7 +101,8 @@ int crux_tree_destroy(crux_tree *tree);
  * an internal node, -2 for a label missing from the alignment) and
  * edge_len[nedges].  Ring and node ids must stay stable across calls so
  * cached CLVs survive; cache validity is re-derived per lnL call exactly as
- * Lik._planRecurse does (Lik.pyx:1500-1532). */
+ * Lik._planRecurse does (Lik.pyx:1500-1532).  The CLV slots of rings that
+ * are detached (ring_node -1) are released. */
 int crux_tree_set_topology(crux_tree *tree, int nnodes, int nedges,
                            const int32_t *ring_node, const int32_t *ring_next,
                            const int32_t *node_ring, const int32_t *node_taxon,
@@ -115,7 +116,9 @@ int crux_tree_set_lengths(crux_tree *tree, int n, const int32_t *edges,
  * of crux_tree_set_topology; node and ring counts are unchanged.  The ring
  * cycles of every node touched are checked as crux_tree_set_topology does.
  * Cached CLVs are kept; their cache keys decide what is recomputed, as after
- * a full topology upload. */
+ * a full topology upload.  The CLV slots of rings left detached are
+ * released (the reference drops a merged edge's cached CLs the same way,
+ * pkg/Crux/Mc3/Chain.pyx:1171-1174). */
 int crux_tree_update_rings(crux_tree *tree, int nrings, const int32_t *rings,
                            const int32_t *ring_node, const int32_t *ring_next,
                            int nnodes, const int32_t *nodes, const int32_t *node_ring);

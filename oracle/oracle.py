@@ -121,6 +121,15 @@ def q_matrix(rtri, pi):
     return Q, pin, qnorm
 
 
+def q_decomp_symmetric(Q, pin):
+    """Eigensystem of a reversible Q through the symmetric Pi^1/2 Q Pi^-1/2:
+    its orthonormal eigenvectors U give V = Pi^-1/2 U and V^-1 = U^T Pi^1/2.
+    Returns (eigenvalues, V, V^-1)."""
+    sq = np.sqrt(pin)
+    w, U = np.linalg.eigh(Q * sq[:, None] / sq[None, :])
+    return w, U / sq[:, None], U.T * sq[None, :]
+
+
 def q_decomp(rtri, pi):
     """(cube[i][j][k] = V[i,k]*Vinv[k,j], eigenvalues, piNorm, qNorm),
     CxLik.c:177-250."""
@@ -128,7 +137,13 @@ def q_decomp(rtri, pi):
     w, V = np.linalg.eig(Q)
     w = np.real(w)
     V = np.real(V)
-    Vi = np.linalg.inv(V)
+    try:
+        Vi = np.linalg.inv(V)
+    except np.linalg.LinAlgError:
+        # The general solver can return linearly dependent eigenvectors for
+        # a repeated eigenvalue of this non-symmetric Q (the reference then
+        # reports a decomposition failure).
+        w, V, Vi = q_decomp_symmetric(Q, pin)
     cube = np.einsum("ik,kj->ijk", V, Vi)
     return np.ascontiguousarray(cube), np.ascontiguousarray(w), pin, qnorm
 

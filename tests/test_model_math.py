@@ -127,3 +127,19 @@ def test_bad_arguments_raise():
         nat.q_decomp([1.0] * 6, [0.25, 0.25, 0.5, 0.0])
     with pytest.raises(ValueError):
         nat.gamma_rates(-1.0, 4)
+
+
+@pytest.mark.parametrize("n", [4, 20])
+def test_oracle_symmetric_decomposition_fallback(n):
+    # The oracle's fallback for a failed general eigensolve: the same P(t)
+    # as scipy's matrix exponential, rows summing to 1.
+    from oracle.oracle import q_decomp_symmetric
+    rng = np.random.default_rng(11 + n)
+    rtri = rng.lognormal(0, 1, n * (n - 1) // 2)
+    pi = rng.dirichlet(np.ones(n))
+    Q, pin, _ = q_matrix(rtri, pi)
+    w, V, Vi = q_decomp_symmetric(Q, pin)
+    assert np.allclose(V @ Vi, np.eye(n), atol=1e-12)
+    for t in (0.05, 0.7):
+        P = V @ np.diag(np.exp(w * t)) @ Vi
+        assert np.allclose(P, scipy.linalg.expm(Q * t), atol=1e-12)
