@@ -47,15 +47,23 @@ def log(*a):
 
 
 def cpu_model():
-    """The host CPU's model name (first processor in /proc/cpuinfo)."""
+    """The host CPU as /proc/cpuinfo names its first processor: model name,
+    family / model / stepping, clock and cache size (a virtualised host may
+    report only a short model name)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for ln in f:
-                if ln.startswith("model name"):
-                    return ln.split(":", 1)[1].strip()
+                if not ln.strip():
+                    break
+                k, _, v = ln.partition(":")
+                info[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    name = info.get("model name")
+    extra = ["%s %s" % (k, info[k]) for k in ("cpu family", "model", "stepping", "cpu MHz",
+                                              "cache size") if k in info]
+    return "%s (%s)" % (name, ", ".join(extra)) if extra else name
 
 
 def available_cores():
