@@ -159,6 +159,14 @@ def bench_dna(args, out):
     ctx.set_timing(False)
     trav_ms = float(np.mean(tr))
     achieved = traffic / (trav_ms * 1e-3) / 1e9
+    # Host side of an evaluation (median over 50, device timing off):
+    # planning, building the batch, launch calls, completion wait + sums.
+    hs = []
+    for _ in range(50):
+        full()
+        hs.append(ctx.last_host_timing().copy())
+    host_us = [round(float(x), 2) for x in np.median(np.array(hs), axis=0)]
+    log("dna host us: plan %.1f build %.1f launch %.1f wait+sum %.1f" % tuple(host_us))
     log("dna traversal %.4f ms, P-matrix %.4f ms, gap %.4f ms: %.1f GB/s" %
         (trav_ms, float(np.mean(pm)), float(np.mean(gap)), achieved))
 
@@ -229,6 +237,7 @@ def bench_dna(args, out):
         "autotune": tune,
         "kernel_ms": {"pmat": round(float(np.mean(pm)), 4), "traverse": round(trav_ms, 4),
                       "launch_gap": round(float(np.mean(gap)), 4)},
+        "host_us": dict(zip(("plan", "build", "launch", "wait_and_sum"), host_us)),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_hbm, "algorithmic_bytes": traffic, "node_updates": nodes,

@@ -83,6 +83,7 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_int)]),
     "crux_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "crux_ctx_last_timing": (ctypes.c_int, [_vp, _dp]),
+    "crux_ctx_last_host_timing": (ctypes.c_int, [_vp, _dp]),
     "crux_ctx_last_traffic": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_int)]),
@@ -242,6 +243,13 @@ class Context:
         out = np.zeros(4)
         check(lib().crux_ctx_last_timing(self.handle, dptr(out)))
         return out
+
+    def last_host_timing(self):
+        """Host microseconds of the last batch: planning, building, launch
+        calls, completion wait + final sums."""
+        v = np.zeros(4)
+        check(lib().crux_ctx_last_host_timing(self.handle, dptr(v)))
+        return v
 
     def last_traffic(self):
         """(bytes moved, bytes under the all-children-from-HBM model, node

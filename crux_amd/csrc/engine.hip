@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -82,6 +83,11 @@ struct crux_ctx {
     float *d_zero = nullptr;       // kZeroBytes of zeros (20/22-state kernel)
     bool timing = false;
     double last_ms[4] = {0, 0, 0, 0};
+    // Host time of the last batch (us): planning, building, launch calls,
+    // completion wait + final sums; plan_us accumulates the planning of the
+    // likelihoods of the batch being formed.
+    double host_us[4] = {0, 0, 0, 0};
+    double plan_us = 0.0;
     uint64_t last_bytes = 0;      // HBM bytes the last evaluation had to move
     uint64_t last_formula = 0;    // same with every child CLV read from HBM
     int last_nodes = 0;
@@ -650,7 +656,23 @@ static int op_shape(const Op &op, const PlanOp &pop)
     return 1 + 4 * src[0] + src[1];
 }
 
+static double us_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static int plan_lik_(crux_lik *L, int root);
+
+// Plans one likelihood; its host time adds to the batch being formed.
 static int plan_lik(crux_lik *L, int root)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = plan_lik_(L, root);
+    L->tree->ctx->plan_us += us_since(t0);
+    return rc;
+}
+
+static int plan_lik_(crux_lik *L, int root)
 {
     crux_tree *t = L->tree;
     L->ops.clear();
@@ -882,6 +904,7 @@ static int finish(crux_ctx *ctx, Staging &S)
 {
     if (!S.pending)
         return CRUX_OK;
+    const auto t0 = std::chrono::steady_clock::now();
     const hipError_t e = hipEventSynchronize(S.done);
     S.pending = false;
     if (e != hipSuccess) {
@@ -922,6 +945,7 @@ static int finish(crux_ctx *ctx, Staging &S)
     ctx->results[S.ticket] = std::move(r);
     S.liks.clear();
     S.items.clear();
+    ctx->host_us[3] = us_since(t0);
     return CRUX_OK;
 }
 
@@ -964,6 +988,9 @@ static int drain(crux_ctx *ctx)
 static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik *> &liks,
                      uint64_t *ticket)
 {
+    const auto t_build = std::chrono::steady_clock::now();
+    ctx->host_us[0] = ctx->plan_us;
+    ctx->plan_us = 0.0;
     // Classes by (K, CM) for the traversal launches.
     struct Item {
         crux_lik *L;
@@ -1320,6 +1347,8 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     }
 
     hipStream_t st = ctx->stream;
+    ctx->host_us[1] = us_since(t_build);
+    const auto t_launch = std::chrono::steady_clock::now();
     HIP_TRY(hipSetDevice(ctx->device));
     // One launch computes the P matrices and copies the plan from the pinned
     // buffer to device memory (no runtime copy: on this ROCm the launch
@@ -1363,6 +1392,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         HIP_TRY(hipMemcpyAsync(S.h_partials, S.d_partials, 2 * tiles * sizeof(double),
                                hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(S.done, st));
+    ctx->host_us[2] = us_since(t_launch);
     ctx->last_bytes = traffic;
     ctx->last_formula = formula;
     ctx->last_nodes = nodes;
@@ -1707,6 +1737,15 @@ int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4)
         return set_error(CRUX_EINVAL, "NULL argument");
     for (int i = 0; i < 4; i++)
         ms4[i] = ctx->last_ms[i];
+    return CRUX_OK;
+}
+
+int crux_ctx_last_host_timing(const crux_ctx *ctx, double *us4)
+{
+    if (!ctx || !us4)
+        return set_error(CRUX_EINVAL, "NULL argument");
+    for (int i = 0; i < 4; i++)
+        us4[i] = ctx->host_us[i];
     return CRUX_OK;
 }
 

@@ -303,6 +303,13 @@ int crux_ctx_launch_shape(const crux_ctx *ctx, int K, int C, int S, int *shape4)
 int crux_ctx_set_timing(crux_ctx *ctx, int on);
 int crux_ctx_last_timing(const crux_ctx *ctx, double *ms4);
 
+/* Host time of the last batch, in microseconds, always recorded: [0]
+ * planning its likelihoods (cache walk and plan emission), [1] building the
+ * batch (stack schedule, control words, P jobs, entries), [2] the launch
+ * calls, [3] waiting for its completion and the final sums.  No counterpart
+ * in the reference. */
+int crux_ctx_last_host_timing(const crux_ctx *ctx, double *us4);
+
 /* Algorithmic HBM traffic of the last evaluation in bytes: `bytes` counts
  * what the kernels must move (CLV and exponent writes, CLVs of children cached
  * by earlier evaluations, tip codes, weights, site lnLs); `formula_bytes`

@@ -1,5 +1,5 @@
 """The measurement hooks the benchmarks rely on: the byte accounting behind
-the roofline figure and the device timing.
+the roofline figure, the device timing and the host timing.
 
 The roofline's algorithmic bytes of a full 4-state traversal are, per the
 byte model, S*C*K*4 + 4S per parent write (CLV plus exponents), S per tip
@@ -74,3 +74,23 @@ def test_device_timing_fields():
     assert pm > 0.0 and trav > 0.0 and gap >= 0.0
     assert total >= trav and total >= pm
     assert abs(total - (pm + gap + trav)) <= 1e-3 + 0.01 * total
+
+
+def test_host_timing_fields():
+    # Host microseconds of the last batch, recorded whether or not device
+    # timing is on: planning, building, launch calls, wait + final sums.
+    say("host timing")
+    prob = synth.make_problem(24, 2000, 9)
+    lik = Lik(prob.tree, prob.alignment, ncat=4)
+    synth.configure(lik, prob.model)
+    ctx = lik._ctx
+    lik.flush()
+    lik.lnL()
+    plan, build, launch, wait = (float(x) for x in ctx.last_host_timing())
+    assert plan > 0.0 and build > 0.0 and launch > 0.0 and wait > 0.0
+    assert plan + build + launch + wait < 1e6
+    # A second batch overwrites the figures.
+    e = list(lik.tree.edges)[3]
+    e.length *= 1.1
+    lik.lnL(e.ring.node)
+    assert float(ctx.last_host_timing()[0]) > 0.0
