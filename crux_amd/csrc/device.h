@@ -244,14 +244,11 @@ struct Entry {
     // update: nchild | kinds << 8 | from_stack << 16 | forwarded << 20 (bit
     // j: child j is the previous update's result, kept in registers) |
     // has_parent << 24 | shape << 25 (see ChildSrc), pos,
-    // tip_load, has_parent, parent (2 words), parent_exp (2 words); for
-    // K != 4 then per child j at 8 + 6 j: P image or tip table, CLV, exponents
-    // (2 words each; the context's zero block where the child has none).  The
-    // tip list (tip_rows, ntips) is per entry, so a one-update entry starts
-    // its own.
+    // tip_load, has_parent; 4 states: then parent (2 words), parent_exp (2
+    // words); K != 4: then buffer descriptors (kCtl20Parent, kCtl20Data,
+    // kCtl20Slot).  The tip list (tip_rows, ntips) is per entry, so a
+    // one-update entry starts its own.
     const int32_t *ctl;
-    // K != 4: a block of zeros that loads without a target are pointed at.
-    const float *zero;
 };
 
 // Child sources named by the control word's update shape: shape
@@ -266,6 +263,17 @@ enum ChildSrc : int {
 
 constexpr int kCtlWords = 8;
 constexpr int kCtlWords20 = 32;
+// 20/22-state control words: words 0-3 as for 4 states (flags and shape,
+// pos, tip_load, has_parent), then gfx950 buffer descriptors (4 words each)
+// that the kernel loads into scalar registers as they are: the parent's CLV
+// slot (CLV, then exponents at exp_off), the P image or tip table of each
+// child, and the CLV slot of each child (internal and identity children).
+constexpr int kCtl20Parent = 4;
+constexpr int kCtl20Data = 8;
+constexpr int kCtl20Slot = 20;
+// Dword 3 of every buffer descriptor (the value __builtin_amdgcn_make_buffer_rsrc
+// is given in the kernels).
+constexpr uint32_t kVsharpFlags = 0x00020000u;
 constexpr size_t kZeroBytes = 64 * 1024;
 
 enum PLayout : int32_t {

@@ -879,6 +879,19 @@ static std::vector<int32_t> assign_tips(std::vector<PlanOp> &ops, bool per_launc
 
 using Staging = crux_ctx::Staging;
 
+// A gfx950 buffer resource descriptor (V#) for raw access to nbytes at p:
+// base address (48 bits), stride 0, no swizzle, range nbytes, and the dword-3
+// flags the kernels' own descriptors use.  Accesses at or past the range read
+// zero and write nothing.
+static void put_vsharp(int32_t *w, const void *p, uint32_t nbytes)
+{
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    w[0] = (int32_t)(uint32_t)a;
+    w[1] = (int32_t)((uint32_t)(a >> 32) & 0xffffu);
+    w[2] = (int32_t)nbytes;
+    w[3] = (int32_t)kVsharpFlags;
+}
+
 // Neumaier addition of (x, xc) into (s, c); an infinite running sum drops
 // the compensation so that -inf stays -inf (the kernels' wave sums do the
 // same).
@@ -1055,6 +1068,12 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         // launch).
         if (!plain)
             schedule_stack(L->ops, cls_shape[class_index(K, it.CM)].depth);
+        // The 20/22-state traversal runs the plan's last update apart from
+        // the others: only that one may lack a parent (it is the root).
+        if (K != 4)
+            for (size_t o = 0; o + 1 < L->ops.size(); o++)
+                if (L->ops[o].parent_slot < 0)
+                    return set_error(CRUX_EINVAL, "plan has a root update before its end");
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
         nops += L->ops.size();
@@ -1204,11 +1223,6 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             op.tip_load = pop.tip_load;
             int32_t *cw = (int32_t *)(H + off_ctl) + ictl;
             ictl += nw;
-            auto put_ptr = [&](int k, const void *p) {
-                const uint64_t v = (uint64_t)(uintptr_t)p;
-                cw[k] = (int32_t)(uint32_t)v;
-                cw[k + 1] = (int32_t)(uint32_t)(v >> 32);
-            };
             auto write_ctl = [&]() {
                 memset(cw, 0, nw * sizeof(int32_t));
                 cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20) |
@@ -1224,14 +1238,29 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                 cw[7] = (int32_t)(uint32_t)(pe >> 32);
                 if (K == 4)
                     return;
-                // Per child: P image or tip table, CLV, exponents; the zero
-                // block where the child has none.
+                // 20/22 states: buffer descriptors the kernel loads into
+                // scalar registers as they are (see Ctl20 in device.h): the
+                // parent's slot, each child's P image or tip table and each
+                // child's slot; the zero block where there is none (an empty
+                // range for slots, so every load of one reads zero).
+                const uint32_t slot_range = (uint32_t)(A.clv_bytes + (size_t)t->S * sizeof(int32_t));
+                put_vsharp(cw + kCtl20Parent, pop.parent_slot >= 0 ? (const void *)A.clv(pop.parent_slot)
+                                                                   : ctx->d_zero,
+                           pop.parent_slot >= 0 ? slot_range : 0u);
                 for (int j = 0; j < kOpMaxChild; j++) {
                     const OpChild &oc = op.ch[j];
                     const bool have = j < op.nchild;
-                    put_ptr(8 + 6 * j, have && oc.mat ? (const void *)oc.mat : ctx->d_zero);
-                    put_ptr(10 + 6 * j, have && oc.clv ? (const void *)oc.clv : ctx->d_zero);
-                    put_ptr(12 + 6 * j, have && oc.exps ? (const void *)oc.exps : ctx->d_zero);
+                    // The data's own size, so that an offset past it reads
+                    // nothing.
+                    const int kind = (op.kinds >> (2 * j)) & 3;
+                    const uint32_t dat_range =
+                        !(have && oc.mat) ? (uint32_t)kZeroBytes
+                        : kind == kChildTip ? (uint32_t)((size_t)t->ncodes * CM * KP * sizeof(float))
+                                            : (uint32_t)(img_words(KP, CM, ctx->prec) * sizeof(float));
+                    put_vsharp(cw + kCtl20Data + 4 * j, have && oc.mat ? (const void *)oc.mat : ctx->d_zero,
+                               dat_range);
+                    put_vsharp(cw + kCtl20Slot + 4 * j, have && oc.clv ? (const void *)oc.clv : ctx->d_zero,
+                               have && oc.clv ? slot_range : 0u);
                 }
             };
             if (pop.parent_slot >= 0) {
@@ -1290,7 +1319,6 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         Entry base{};
         base.ops = dops + op0;
         base.ctl = (const int32_t *)(D + off_ctl) + ctl0;
-        base.zero = ctx->d_zero;
         base.nops = (int32_t)L->ops.size();
         base.S = t->S;
         base.C = C;

@@ -271,6 +271,10 @@ __device__ __forceinline__ int32_t bload1(__amdgpu_buffer_rsrc_t r, uint32_t vof
     return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
 }
 
+// s_waitcnt immediate that waits for LDS and scalar loads only (vmcnt and
+// expcnt at their maxima, lgkmcnt 0).
+constexpr int kWaitLgkm0 = 0xC07F;
+
 // Cache policy of a store: 0, or kNT (the non-temporal bit of gfx950's
 // buffer instructions) for CLV stores, which a later evaluation reads back,
 // if at all, long after the caches have turned over.
@@ -1224,15 +1228,14 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 // 20/22-state traversal.  A block is one wave covering CM mixture components
 // of G = 64 / CM consecutive sites: lane l holds component l / G of site
 // l % G, all KP states of it in registers (states >= K stay 0).  Like the
-// 4-state kernel, the wave walks the whole plan for its sites with no
-// barrier, results of an update are stored to their HBM slot ([S][C][KP])
-// and handed to the consumer in registers (the update just before it) or
-// through the wave's LDS stack, and the root update reduces to the
-// pattern-weighted FP64 site lnL (NaN -> -inf, CxLik.c:598-632) and a
-// compensated sum per wave.  A slot holds the wave's registers as they are,
-// [S / G][KP / 4][64 lanes][4] (lanes of components >= C unused), so every
-// store and load of a CLV moves 1 KiB contiguously.  Arithmetic follows the
-// generic stripe
+// 4-state kernels, the wave walks the whole plan for its sites with no
+// barrier; results of an update are stored to their HBM slot and handed to
+// the consumer in registers (the update just before it) or through the
+// wave's LDS stack, and the root update reduces to the pattern-weighted FP64
+// site lnL (NaN -> -inf, CxLik.c:598-632) and a compensated sum per wave.  A
+// slot holds the wave's registers as they are, [S / G][KP / 4][64 lanes][4]
+// (lanes of components >= C unused), so every store and load of a CLV moves
+// 1 KiB contiguously.  Arithmetic follows the generic stripe
 // CxLikExecuteStripe (CxLik.c:635-846) with FP32 storage and power-of-two
 // scaling.
 //
@@ -1245,20 +1248,49 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 // k_pmat, one float per register, so an update's P for all components is a
 // few coalesced float4 loads per lane.  A tip child is a row of the tip
 // table (P_c times the leaf vector of the site's code; the codes of up to
-// kTipChunk tips are staged in LDS).  Everything a child needs from memory
-// -- P image or tip row, and the CLV of a child cached by an earlier plan --
-// is loaded one update ahead: right after child j of update o is consumed,
-// the loads for child j of update o + 1 are issued into the same registers,
-// so they are in flight while the rest of update o runs.  The control words
-// of an update are loaded two updates ahead (one dword per lane, read out
-// with readlane).  A third child (the root update and chained
-// multifurcations) is loaded where it is used.
+// kTipChunk tips are staged in LDS).
+//
+// A wave walks its plan alone on its SIMD more often than not (a launch has
+// about one wave per SIMD at 20k patterns), so an update costs the wave's
+// instruction count plus what it waits for; the structure keeps both small:
+// * Control words are host-built records whose buffer descriptors load
+//   straight into scalar registers (s_load; no per-lane decoding).  The
+//   records of updates o + 1 and o + 2 are loaded when update o ends, once
+//   every LDS read of update o has returned (scalar loads return out of
+//   order and share a counter with LDS reads, so any wait for one of those
+//   waits for them too), and waited for when update o + 1 starts.
+// * Updates with two children and a parent (all but the root, chained
+//   multifurcations, identity children and results re-read after the stack
+//   overflowed) run a body specialised for their children's sources (tip,
+//   forwarded, stack, cached in HBM by an earlier plan), named by the
+//   update's shape code; other updates take a generic walk that loads what
+//   it needs where it needs it.
+// * What a child needs from memory is loaded one update ahead: a tip child's
+//   tip row, a forwarded or stack child's P image, a cached child's CLV (its
+//   P image is loaded in the update itself, from L2).  The loads are issued
+//   at one place after the current update's products, the same loads for
+//   every kind of child with only descriptor and offsets chosen by mask
+//   arithmetic: loads issued on several paths, or inside the bodies, leave
+//   the register allocator copying their targets where the paths join, and
+//   a copy waits for its load.  The next update's tip codes are read from
+//   LDS when the current update starts.
+// * The plan's last update (its root) runs after the loop, and nothing
+//   loaded before the loop stays outstanding into it, so that every way into
+//   a wait in the loop has the same stores after the loads it waits for (a
+//   wait count is the least over all ways in, and one counter tracks loads
+//   and stores in order).
+//
+// Idle lanes (components >= C) hold zeros throughout: their P image rows and
+// tip rows are zero, and their accesses to CLV slots use an offset outside
+// every buffer range (loads read zero, stores write nothing).
+//
 // WV: waves per SIMD the register allocation is held to (1: up to 512
 // registers per lane, 2: up to 256).
 template <int KP, int CM, int D, int PREC, int WV>
 __global__ void __launch_bounds__(64, WV)
 k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
 {
+    using rsrc_t = __amdgpu_buffer_rsrc_t;
     constexpr int G = sites20(CM);                  // sites per wave
     constexpr int W = img_cols(CM);                 // P columns per image register
     constexpr int CBSZ = W == 16 ? 4 : W == 8 ? 3 : W == 4 ? 2 : W == 2 ? 1 : 0;
@@ -1273,6 +1305,9 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     constexpr int PQ = NQ > NI ? NQ : NI;           // float4 of a prefetched child
     constexpr int WPR = G / 4;                      // code dwords per staged tip row
     constexpr int RPI = 64 / WPR;                   // tip rows per staging load
+    // Byte offset no buffer range covers (the largest offset added to it
+    // stays below 2^32).
+    constexpr uint32_t kOut = 0x80000000u;
     static_assert((1 << CBSZ) == W, "component lane groups are whole matrix blocks");
     __shared__ crux_f32x4 stk[D][NI][64];
     __shared__ int32_t stke[D][64];
@@ -1281,22 +1316,64 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     const int b = blk0 + (int)blockIdx.x;
     const CRUX_CAS Entry *E = cas(entries) + cas(block_entry)[b];
     const int S = E->S, C = E->C, K = E->K, nops = E->nops, ntips = E->ntips;
+    const uint32_t exp_off = (uint32_t)E->exp_off;
     const int tile = b - E->tile0;
     const int lane = (int)threadIdx.x;
     const int c = lane / G, t = lane % G;
     const int s = tile * G + t;
     const bool site_ok = s < S;
     const int sc = site_ok ? s : S - 1;     // in-bounds site for loads of idle lanes
-    const CRUX_GAS int32_t *ctl = gas(E->ctl);
-    const char *zero = reinterpret_cast<const char *>(E->zero);
+    const uint32_t lane16 = (uint32_t)lane * 16;
+    // Offsets within a CLV slot: the lane's first float4 (the others follow
+    // 1 KiB apart; kOut for idle components); its site's exponent, stored
+    // from component 0 of sites in range only.
+    const uint32_t vcl = c < C ? (uint32_t)tile * NI * 1024 + lane16 : kOut;
+    const uint32_t vexs = (c == 0 && site_ok) ? exp_off + (uint32_t)s * 4 : kOut;
+    const uint32_t vexl = exp_off + (uint32_t)sc * 4;
+    // A tip row of the lane's component: code * CM * KP floats further.
+    const uint32_t vtip = (uint32_t)c * KP * 4;
+    const CRUX_CAS uint32_t *ctlw = (const CRUX_CAS uint32_t *)cas(E->ctl);
+    const uint8_t *tipb = reinterpret_cast<const uint8_t *>(&tipw[0][0]);
     double sum = 0.0;
 
-    auto word = [](int32_t cw, int k) -> int32_t { return __builtin_amdgcn_readlane(cw, k); };
-    auto ptr = [&](int32_t cw, int k) -> const char * {
-        const uint64_t lo = (uint32_t)word(cw, k), hi = (uint32_t)word(cw, k + 1);
-        return reinterpret_cast<const char *>(lo | (hi << 32));
+    // One update's control record (scalar registers): flags, positions,
+    // tip_load, has_parent, and of each buffer descriptor its base and range
+    // (the descriptor is rebuilt from them where it is used, so that a
+    // choice between two of them is two scalar selects).
+    struct Rec {
+        uint32_t w[kCtl20Slot + 8];
+        __device__ uint32_t w0() const { return w[0]; }
+        __device__ uint32_t pk() const { return w[1]; }
+        __device__ uint32_t tl() const { return w[2]; }
+        __device__ uint32_t hp() const { return w[3]; }
+        // Base and range words of the parent's, child j's data and child j's
+        // slot descriptors.
+        __device__ const uint32_t (&par() const)[3] { return *reinterpret_cast<const uint32_t(*)[3]>(w + kCtl20Parent); }
+        __device__ const uint32_t (&dat(int j) const)[3] { return *reinterpret_cast<const uint32_t(*)[3]>(w + kCtl20Data + 4 * j); }
+        __device__ const uint32_t (&slot(int j) const)[3] { return *reinterpret_cast<const uint32_t(*)[3]>(w + kCtl20Slot + 4 * j); }
     };
-    auto ctl_of = [&](int o) -> int32_t { return ctl[o * kCtlWords20 + (lane & 31)]; };
+    // Words 0 .. kCtl20Slot + 7 of update o's record, in two wide loads.
+    auto rec = [&](int o) -> Rec {
+        const CRUX_CAS uint32_t *w = ctlw + (size_t)o * kCtlWords20;
+        Rec r;
+#pragma unroll
+        for (int k = 0; k < kCtl20Slot + 8; ++k)
+            r.w[k] = w[k];
+        return r;
+    };
+    auto mk = [](const uint32_t (&v)[3]) -> rsrc_t {
+        const uint64_t base = (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+        return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(base), (short)0, (int)v[2],
+                                                 (int)kVsharpFlags);
+    };
+    auto dat_of = [&](int o, int j) -> rsrc_t {
+        return ((const CRUX_CAS rsrc_t *)(ctlw + (size_t)o * kCtlWords20))[kCtl20Data / 4 + j];
+    };
+    auto slot_of = [&](int o, int j) -> rsrc_t {
+        return ((const CRUX_CAS rsrc_t *)(ctlw + (size_t)o * kCtlWords20))[kCtl20Slot / 4 + j];
+    };
+    auto shape_of = [](uint32_t w0) -> int { return (int)((w0 >> 25) & 31); };
+    auto src_of = [](int sh, int j) -> int { return j == 0 ? (sh - 1) >> 2 : (sh - 1) & 3; };
 
     // Codes of tips [base, base + kTipChunk) of the entry's list, for this
     // wave's sites (rows are padded to whole 64-site tiles).
@@ -1321,42 +1398,31 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         }
         __builtin_amdgcn_s_waitcnt(0);
     };
+    // Site code of the tip in row pos of the staged chunk (child j's byte of
+    // the packed positions).
+    auto code_of = [&](uint32_t pk, int j) -> uint32_t {
+        return tipb[((pk >> (8 * j)) & (kTipChunk - 1)) * G + t];
+    };
 
-    // Loads for child j of the update with control words cw (ok: the update
-    // exists).  pr: the P image (internal child) or the tip-table row (tip);
-    // hx/he: the child's CLV and exponent when it comes from HBM.  Every load
-    // is issued whatever the child is -- missing targets read the zero
-    // block -- so no register is merged across branches.
-    const uint32_t lane16 = (uint32_t)lane * 16;
-    const uint8_t *tipb = reinterpret_cast<const uint8_t *>(&tipw[0][0]);
-    auto fetch = [&](int32_t cw, bool ok, int j, crux_f32x4 (&pr)[PQ], crux_f32x4 (&hxr)[NI],
-                     int32_t &her) {
-        const int w0 = ok ? word(cw, 0) : 0;
-        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, via = ((w0 >> 16) | (w0 >> 20)) & 0xf;
-        const bool have = j < nch;
-        const int kind = have ? (kinds >> (2 * j)) & 3 : (int)kChildIdent;
-        const bool hbm = have && kind != kChildTip && !((via >> j) & 1);
-        const bool tip = kind == kChildTip;
-        const char *mat = have && kind != kChildIdent ? ptr(cw, 8 + 6 * j) : zero;
-        const char *clv = hbm ? ptr(cw, 10 + 6 * j) + (size_t)tile * NI * 1024 : zero;
-        const char *exs = hbm ? ptr(cw, 12 + 6 * j) : zero;
-        const int pos = op_pos(ok ? word(cw, 1) : 0, j) & (kTipChunk - 1);
-        const uint32_t code = tipb[pos * G + t];
-        // float4 q of the lane: a tip row is NI consecutive float4 at the
-        // row of its (code, component); an image NQ float4 a wave apart.
-        const uint32_t voff = tip ? (code * CM + c) * (KP * 4) : lane16;
-        const auto rm = rsrc_of(mat), rh = rsrc_of(clv), re = rsrc_of(exs);
+    // Loads: a P image (PQ float4, a wave apart), a tip row (NI consecutive
+    // float4 of the lane's (code, component) row), a CLV (NI float4 1 KiB
+    // apart) and its exponent.
+    auto load_img = [&](rsrc_t r, crux_f32x4 (&pr)[PQ]) {
 #pragma unroll
-        for (int q = 0; q < PQ; ++q)
-            pr[q] = bload4(rm, voff, tip ? (q < NI ? q : NI - 1) * 16 : (q < NQ ? q : NQ - 1) * 1024);
-        // Only a child cached in HBM has a CLV to load (the registers keep
-        // whatever they held otherwise; nothing reads them then).
-        if (hbm) {
+        for (int q = 0; q < NQ; ++q)
+            pr[q] = bload4(r, lane16, q * 1024);
+    };
+    auto load_tip = [&](rsrc_t r, uint32_t code, crux_f32x4 (&pr)[PQ]) {
+        const uint32_t voff = code * (CM * KP * 4) + vtip;
 #pragma unroll
-            for (int q = 0; q < NI; ++q)
-                hxr[q] = bload4(rh, lane16, q * 1024);
-            her = bload1(re, (uint32_t)sc * 4, 0);
-        }
+        for (int q = 0; q < NI; ++q)
+            pr[q] = bload4(r, voff, q * 16);
+    };
+    auto load_clv = [&](rsrc_t r, crux_f32x4 (&x)[NI], int32_t &e) {
+#pragma unroll
+        for (int q = 0; q < NI; ++q)
+            x[q] = bload4(r, vcl, q * 1024);
+        e = bload1(r, vexl, 0);
     };
 
     // 16-bit products: four columns per instruction, the lane's own four
@@ -1378,13 +1444,12 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         return r;
     };
     auto pmul16 = [&](const crux_f32x4 (&img)[PQ], const crux_f32x4 (&x)[NI], crux_f32x4 (&y)[NI],
-                      const char *mat) {
+                      rsrc_t mat) {
         crux_f32x4 lo[PREC == kPrecF16x3 ? NV16P / 4 : 1];
         if constexpr (PREC == kPrecF16x3) {
-            const auto rl = rsrc_of(mat);
 #pragma unroll
             for (int q = 0; q < NV16P / 4; ++q)
-                lo[q] = bload4(rl, (uint32_t)lane * 16, (uint32_t)(NV16P / 4 + q) * 1024);
+                lo[q] = bload4(mat, lane16, (uint32_t)(NV16P / 4 + q) * 1024);
         }
         auto loword = [&](int v) -> uint32_t { return __float_as_uint(lo[v >> 2][v & 3]); };
 #pragma unroll
@@ -1431,12 +1496,14 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         }
     };
 
-    // y = P_c x for every lane's own (site, component).
+    // y = P_c x for every lane's own (site, component); mat: the P image's
+    // descriptor (the split precision's low image is read from it).
     auto pmul = [&](const crux_f32x4 (&img)[PQ], const crux_f32x4 (&x)[NI], crux_f32x4 (&y)[NI],
-                    const char *mat) {
+                    rsrc_t mat) {
         if constexpr (PREC != kPrecF32) {
             pmul16(img, x, y, mat);
         } else {
+            (void)mat;
 #pragma unroll
             for (int I = 0; I < NI; ++I)
                 y[I] = crux_f32x4{0.f, 0.f, 0.f, 0.f};
@@ -1458,15 +1525,10 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         }
     };
 
+    // What the current update's children 0 and 1 took from memory ahead of
+    // it (shape updates): P image, tip row, or CLV and exponent.
     crux_f32x4 pre[2][PQ];
-    crux_f32x4 hx[2][NI];
-    int32_t he[2];
-    int32_t cw0 = ctl_of(0), cw1 = ctl_of(min(1, nops - 1));
-    if (ntips > 0)
-        stage_tips(0);
-    fetch(cw0, true, 0, pre[0], hx[0], he[0]);
-    fetch(cw0, true, 1, pre[1], hx[1], he[1]);
-
+    int32_t he[2] = {0, 0};
     // The previous update's scaled result and exponent (forwarded children).
     crux_f32x4 yl[NI];
     int32_t el = 0;
@@ -1474,121 +1536,289 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     for (int I = 0; I < NI; ++I)
         yl[I] = crux_f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int o = 0; o < nops; ++o) {
-        const bool nxt = o + 1 < nops;
-        const int32_t cw2 = ctl_of(min(o + 2, nops - 1));
-        const int w0 = word(cw0, 0);
-        const int32_t pk = word(cw0, 1);
-        const bool has_parent = word(cw0, 3) != 0;
+    // Child j of the next update (record rn, shape nsh != 0, tip code cd):
+    // the same PQ float4 and exponent loads whatever the child is, only the
+    // descriptor and offsets differ (so every way through writes the same
+    // registers, and none of them is copied, which would wait for the
+    // loads).
+    auto fetch_next = [&](const Rec &rn, int nsh, uint32_t cd, int j) {
+        const int src = src_of(nsh, j);
+        const bool tip = src == kSrcTip, hbm = src == kSrcHbm;
+        uint32_t v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            v[k] = hbm ? rn.slot(j)[k] : rn.dat(j)[k];
+        const rsrc_t r = mk(v);
+        // Selects by mask arithmetic, which the compiler keeps as such (a
+        // select it turned into a branch would split the update's body).
+        const uint32_t mtip = tip ? ~0u : 0u, mhbm = hbm ? ~0u : 0u;
+        const uint32_t vimg = lane16 ^ ((lane16 ^ vcl) & mhbm);
+        const uint32_t vrow = cd * (CM * KP * 4) + vtip;
+        const uint32_t voff = vimg ^ ((vimg ^ vrow) & mtip);
+        // Float4 past NI of a tip row or CLV: an offset outside the range
+        // (no memory request).
+        const uint32_t voff_hi = voff | ((mtip | mhbm) & kOut);
+        const int sh = tip ? 4 : 10;
+#pragma unroll
+        for (int q = 0; q < PQ; ++q)
+            pre[j][q] = bload4(r, q < NI ? voff : voff_hi, (uint32_t)q << sh);
+        // The exponent of a cached child (an empty range otherwise).
+        uint32_t ve[3] = {rn.slot(j)[0], rn.slot(j)[1], hbm ? rn.slot(j)[2] : 0u};
+        he[j] = bload1(mk(ve), vexl, 0);
+    };
+
+    // Generic walk of update o (any child count and kind): every input is
+    // loaded where it is used; cdc: the site codes of its tip children.
+    auto generic = [&](int o, const Rec &r, const uint32_t (&cdc)[kOpMaxChild], crux_f32x4 (&acc)[NI],
+                       int32_t &ex) {
+        const int w0 = (int)r.w0();
         const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xf,
                   fwd = (w0 >> 20) & 0xf;
-        // The next update's tips may start a new chunk; this update's tip
-        // rows are already in registers.
-        if (nxt) {
-            const int tl = word(cw1, 2);
-            if (tl > 0)
-                stage_tips(tl);
-        }
-        crux_f32x4 acc[NI];
-        int32_t ex = 0;
         static_for<kOpMaxChild>([&](auto J) {
             constexpr int j = decltype(J)::value;
-            if (j < nch) {
-                const int kind = (kinds >> (2 * j)) & 3;
-                crux_f32x4 p2[j < 2 ? 1 : PQ], h2[j < 2 ? 1 : NI];
-                int32_t e2x = 0;
-                if constexpr (j == 2)
-                    fetch(cw0, true, 2, p2, h2, e2x);
-                auto &pj = [&]() -> auto & {
-                    if constexpr (j < 2)
-                        return pre[j];
-                    else
-                        return p2;
-                }();
-                auto &hj = [&]() -> auto & {
-                    if constexpr (j < 2)
-                        return hx[j];
-                    else
-                        return h2;
-                }();
-                const int32_t hej = j < 2 ? he[j < 2 ? j : 0] : e2x;
-                crux_f32x4 y[NI];
-                if (kind == kChildTip) {
+            if (j >= nch)
+                return;
+            const int kind = (kinds >> (2 * j)) & 3;
+            const rsrc_t mat = dat_of(o, j);
+            crux_f32x4 y[NI];
+            crux_f32x4 pr[PQ];
+            if (kind == kChildTip) {
+                load_tip(mat, cdc[j], pr);
+#pragma unroll
+                for (int I = 0; I < NI; ++I)
+                    y[I] = pr[I];
+            } else {
+                crux_f32x4 x[NI];
+                if ((fwd >> j) & 1) {
 #pragma unroll
                     for (int I = 0; I < NI; ++I)
-                        y[I] = pj[I];
+                        x[I] = yl[I];
+                    ex += el;
+                } else if ((fst >> j) & 1) {
+                    const int p = op_pos((int32_t)r.pk(), j) & (D - 1);
+#pragma unroll
+                    for (int I = 0; I < NI; ++I)
+                        x[I] = stk[p][I][lane];
+                    ex += stke[p][lane];
                 } else {
-                    crux_f32x4 x[NI];
-                    if ((fwd >> j) & 1) {
-#pragma unroll
-                        for (int I = 0; I < NI; ++I)
-                            x[I] = yl[I];
-                        ex += el;
-                    } else if ((fst >> j) & 1) {
-                        const int p = op_pos(pk, j) & (D - 1);
-#pragma unroll
-                        for (int I = 0; I < NI; ++I)
-                            x[I] = stk[p][I][lane];
-                        ex += stke[p][lane];
-                    } else {
-#pragma unroll
-                        for (int I = 0; I < NI; ++I)
-                            x[I] = hj[I];
-                        ex += hej;
-                    }
-                    if (kind == kChildIdent) {
-                        // Used as is; idle components hold no CLV.
-#pragma unroll
-                        for (int I = 0; I < NI; ++I)
-                            y[I] = c < C ? x[I] : crux_f32x4{0.f, 0.f, 0.f, 0.f};
-                    } else {
-                        pmul(pj, x, y, ptr(cw0, 8 + 6 * j));
-                    }
+                    int32_t e = 0;
+                    load_clv(slot_of(o, j), x, e);
+                    ex += e;
                 }
+                if (kind == kChildIdent) {
+                    // Used as is (idle components hold zeros).
 #pragma unroll
-                for (int I = 0; I < NI; ++I)
-                    acc[I] = j == 0 ? y[I] : acc[I] * y[I];
+                    for (int I = 0; I < NI; ++I)
+                        y[I] = x[I];
+                } else {
+                    load_img(mat, pr);
+                    pmul(pr, x, y, mat);
+                }
             }
-            // Child j's registers are free: load child j of the next update.
-            if constexpr (j < 2)
-                fetch(cw1, nxt, j, pre[j], hx[j], he[j]);
+#pragma unroll
+            for (int I = 0; I < NI; ++I)
+                acc[I] = j == 0 ? y[I] : acc[I] * y[I];
         });
+    };
 
-        if (has_parent) {
-            // Site maximum over the lane's states, then over the site's
-            // components (lanes G apart); all values are >= 0, so the
-            // largest signed integer is the largest float.
-            int32_t mb = 0;
+    // Site maximum over the lane's states, then over the site's components
+    // (lanes G apart); all values are >= 0, so the largest signed integer is
+    // the largest float.  The result is scaled into [0.5, 1) by an exact
+    // power of two, its exponent added to the children's.
+    auto rescale = [&](const crux_f32x4 (&acc)[NI], int32_t ex) {
+        int32_t mb = 0;
+#pragma unroll
+        for (int I = 0; I < NI; ++I)
+            mb = max(max(mb, max(__float_as_int(acc[I].x), __float_as_int(acc[I].y))),
+                     max(__float_as_int(acc[I].z), __float_as_int(acc[I].w)));
+        mb = comp_max<G>(mb);
+        const float m = __int_as_float(mb);
+        int e2 = 0;
+        if (m > 0.0f)
+            (void)frexpf(m, &e2);
+        el = ex + e2;
+#pragma unroll
+        for (int I = 0; I < NI; ++I)
+            yl[I] = crux_f32x4{ldexpf(acc[I].x, -e2), ldexpf(acc[I].y, -e2), ldexpf(acc[I].z, -e2),
+                               ldexpf(acc[I].w, -e2)};
+    };
+    // The result to the parent's slot (one float4 of every lane per store:
+    // 1 KiB contiguous) and, if the plan says so, onto the stack.
+    auto store = [&](rsrc_t par, uint32_t pk) {
+#pragma unroll
+        for (int I = 0; I < NI; ++I)
+            bstore4<kNT>(yl[I], par, vcl, I * 1024);
+        bstore1<kNT>(el, par, vexs, 0);
+        const int pp = op_pos((int32_t)pk, kOpMaxChild);
+        if (pp >= 0) {
 #pragma unroll
             for (int I = 0; I < NI; ++I)
-                mb = max(max(mb, max(__float_as_int(acc[I].x), __float_as_int(acc[I].y))),
-                         max(__float_as_int(acc[I].z), __float_as_int(acc[I].w)));
-            mb = comp_max<G>(mb);
-            const float m = __int_as_float(mb);
-            int e2 = 0;
-            if (m > 0.0f)
-                (void)frexpf(m, &e2);
-            el = ex + e2;
-#pragma unroll
-            for (int I = 0; I < NI; ++I)
-                yl[I] = crux_f32x4{ldexpf(acc[I].x, -e2), ldexpf(acc[I].y, -e2), ldexpf(acc[I].z, -e2),
-                                   ldexpf(acc[I].w, -e2)};
-            const int pp = op_pos(pk, kOpMaxChild);
-            if (pp >= 0) {
+                stk[pp][I][lane] = yl[I];
+            stke[pp][lane] = el;
+        }
+    };
+
+    // Two-child update with a parent, children from sources s0 and s1: the
+    // forwarded child's product first (its input is ready), then the stack's
+    // (its LDS reads were issued at the start), then a cached one (its P
+    // image is loaded at the start); a tip child is its row.  Child j's
+    // registers take the next update's child j as soon as they are free.
+    // acc = y0 y1 as in the generic walk.
+    auto two = [&](auto S0c, auto S1c, const Rec &r, const Rec &rn, int nsh, const uint32_t (&cd)[2],
+                   crux_f32x4 (&acc)[NI], int32_t &ex) {
+        constexpr int s0 = decltype(S0c)::value, s1 = decltype(S1c)::value;
+        crux_f32x4 xs[2][NI];
+        int32_t es[2] = {0, 0};
+        crux_f32x4 hi[2][PQ];
+        static_for<2>([&](auto Jc) {
+            constexpr int j = decltype(Jc)::value;
+            constexpr int sj = j == 0 ? s0 : s1;
+            if constexpr (sj == kSrcStk) {
+                const int p = op_pos((int32_t)r.pk(), j) & (D - 1);
 #pragma unroll
                 for (int I = 0; I < NI; ++I)
-                    stk[pp][I][lane] = yl[I];
-                stke[pp][lane] = el;
-            }
-            // One float4 of every lane per store: 1 KiB contiguous.
-            if (c < C) {
-                const auto rd = rsrc_of(ptr(cw0, 4) + (size_t)tile * NI * 1024);
+                    xs[j][I] = stk[p][I][lane];
+                es[j] = stke[p][lane];
+            } else if constexpr (sj == kSrcHbm) {
+                // The cached child's CLV came ahead; its P image now.
+                load_img(mk(r.dat(j)), hi[j]);
 #pragma unroll
                 for (int I = 0; I < NI; ++I)
-                    bstore4<kNT>(yl[I], rd, lane16, I * 1024);
-                if (c == 0 && site_ok)
-                    bstore1<kNT>(el, rsrc_of(ptr(cw0, 6)), (uint32_t)t * 4, (uint32_t)tile * G * 4);
+                    xs[j][I] = pre[j][I];
+                es[j] = he[j];
             }
+        });
+        crux_f32x4 y[2][NI];
+        auto child = [&](auto Jc) {
+            constexpr int j = decltype(Jc)::value;
+            constexpr int sj = j == 0 ? s0 : s1;
+            if constexpr (sj == kSrcTip) {
+#pragma unroll
+                for (int I = 0; I < NI; ++I)
+                    y[j][I] = pre[j][I];
+            } else if constexpr (sj == kSrcFwd) {
+                pmul(pre[j], yl, y[j], mk(r.dat(j)));
+                ex += el;
+            } else if constexpr (sj == kSrcStk) {
+                pmul(pre[j], xs[j], y[j], mk(r.dat(j)));
+                ex += es[j];
+            } else {
+                pmul(hi[j], xs[j], y[j], mk(r.dat(j)));
+                ex += es[j];
+            }
+        };
+        static_for<4>([&](auto Rc) {
+            // Source order: forwarded, tip, stack, cached.
+            constexpr int q = decltype(Rc)::value;
+            constexpr int want = q == 0 ? kSrcFwd : q == 1 ? kSrcTip : q == 2 ? kSrcStk : kSrcHbm;
+            if constexpr (s0 == want)
+                child(std::integral_constant<int, 0>{});
+            if constexpr (s1 == want)
+                child(std::integral_constant<int, 1>{});
+        });
+#pragma unroll
+        for (int I = 0; I < NI; ++I)
+            acc[I] = y[0][I] * y[1][I];
+    };
+    using Tip = std::integral_constant<int, kSrcTip>;
+    using Fwd = std::integral_constant<int, kSrcFwd>;
+    using Stk = std::integral_constant<int, kSrcStk>;
+    using Hbm = std::integral_constant<int, kSrcHbm>;
+    auto two_of = [&](auto S0c, int s1, const Rec &r, const Rec &rn, int nsh, const uint32_t (&cd)[2],
+                      crux_f32x4 (&acc)[NI], int32_t &ex) {
+        if (s1 == kSrcTip)
+            two(S0c, Tip{}, r, rn, nsh, cd, acc, ex);
+        else if (s1 == kSrcStk)
+            two(S0c, Stk{}, r, rn, nsh, cd, acc, ex);
+        else if (s1 == kSrcHbm)
+            two(S0c, Hbm{}, r, rn, nsh, cd, acc, ex);
+        else if constexpr (decltype(S0c)::value != kSrcFwd)
+            two(S0c, Fwd{}, r, rn, nsh, cd, acc, ex);
+    };
+
+    Rec r0 = rec(0);
+    Rec r1 = rec(min(1, nops - 1));
+    if (ntips > 0)
+        stage_tips(0);
+    if (nops > 1) {
+        const int sh = shape_of(r0.w0());
+        if (sh != 0) {
+            const uint32_t cd[2] = {code_of(r0.pk(), 0), code_of(r0.pk(), 1)};
+            fetch_next(r0, sh, cd[0], 0);
+            fetch_next(r0, sh, cd[1], 1);
+        }
+    }
+    // Nothing loaded before the loop stays outstanding into it (see above).
+    __builtin_amdgcn_s_waitcnt(0);
+
+    // Every update but the last has a parent (the last is the plan's root
+    // update, or the one update of a one-update entry).
+    for (int o = 0; o + 1 < nops; ++o) {
+        const int shape = shape_of(r0.w0());
+        // A generic update reads its tip codes before the next update's tips
+        // may replace the staged chunk (a shape update's tip rows are in
+        // registers already).
+        uint32_t cdc[kOpMaxChild] = {0u, 0u, 0u};
+        if (shape == 0) {
+#pragma unroll
+            for (int j = 0; j < kOpMaxChild; ++j)
+                cdc[j] = code_of(r0.pk(), j);
+        }
+        // The next update: staged tips (its tip rows may start a new chunk),
+        // shape (0 unless it is a shape update ahead of the last) and tip
+        // codes.
+        if ((int)r1.tl() > 0)
+            stage_tips((int)r1.tl());
+        const int nsh = o + 2 < nops ? shape_of(r1.w0()) : 0;
+        const uint32_t cd[2] = {code_of(r1.pk(), 0), code_of(r1.pk(), 1)};
+        crux_f32x4 acc[NI];
+        int32_t ex = 0;
+        if (shape != 0) {
+            const int s0 = src_of(shape, 0), s1 = src_of(shape, 1);
+            if (s0 == kSrcTip)
+                two_of(Tip{}, s1, r0, r1, nsh, cd, acc, ex);
+            else if (s0 == kSrcFwd)
+                two_of(Fwd{}, s1, r0, r1, nsh, cd, acc, ex);
+            else if (s0 == kSrcStk)
+                two_of(Stk{}, s1, r0, r1, nsh, cd, acc, ex);
+            else
+                two_of(Hbm{}, s1, r0, r1, nsh, cd, acc, ex);
+        } else {
+            generic(o, r0, cdc, acc, ex);
+        }
+        // The next update's children, from one place for every kind of
+        // update (the registers then need no copying where the paths join).
+        if (nsh != 0) {
+            fetch_next(r1, nsh, cd[0], 0);
+            fetch_next(r1, nsh, cd[1], 1);
+        }
+        // Every LDS read of this update has returned by now; then the record
+        // after next, waited for when the next update starts (nothing from
+        // here to there waits for a scalar or LDS load, and scalar loads
+        // return out of order, so any such wait would wait for it too).
+        // Both records are loaded again (the next one from the scalar cache)
+        // rather than moved: a register copy would wait for the load.
+        const rsrc_t par = mk(r0.par());
+        const uint32_t pk = r0.pk();
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        r0 = rec(o + 1);
+        r1 = rec(min(o + 2, nops - 1));
+        rescale(acc, ex);
+        store(par, pk);
+    }
+
+    // The last update: the root reduction (or one update with a parent).
+    {
+        crux_f32x4 acc[NI];
+        int32_t ex = 0;
+        uint32_t cdc[kOpMaxChild];
+#pragma unroll
+        for (int j = 0; j < kOpMaxChild; ++j)
+            cdc[j] = code_of(r0.pk(), j);
+        generic(nops - 1, r0, cdc, acc, ex);
+        if (r0.hp() != 0) {
+            rescale(acc, ex);
+            store(mk(r0.par()), r0.pk());
         } else {
             // Weights of the lane's component (all loads issued at once;
             // idle components read component 0's and add nothing).
@@ -1611,8 +1841,6 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 sum = v;
             }
         }
-        cw0 = cw1;
-        cw1 = cw2;
     }
     double cmp = 0.0;
 #pragma unroll
