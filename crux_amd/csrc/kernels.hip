@@ -1661,11 +1661,9 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     // Two-child update with a parent, children from sources s0 and s1: the
     // forwarded child's product first (its input is ready), then the stack's
     // (its LDS reads were issued at the start), then a cached one (its P
-    // image is loaded at the start); a tip child is its row.  Child j's
-    // registers take the next update's child j as soon as they are free.
-    // acc = y0 y1 as in the generic walk.
-    auto two = [&](auto S0c, auto S1c, const Rec &r, const Rec &rn, int nsh, const uint32_t (&cd)[2],
-                   crux_f32x4 (&acc)[NI], int32_t &ex) {
+    // image is loaded at the start); a tip child is its row.  acc = y0 y1 as
+    // in the generic walk.
+    auto two = [&](auto S0c, auto S1c, const Rec &r, crux_f32x4 (&acc)[NI], int32_t &ex) {
         constexpr int s0 = decltype(S0c)::value, s1 = decltype(S1c)::value;
         crux_f32x4 xs[2][NI];
         int32_t es[2] = {0, 0};
@@ -1724,16 +1722,15 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     using Fwd = std::integral_constant<int, kSrcFwd>;
     using Stk = std::integral_constant<int, kSrcStk>;
     using Hbm = std::integral_constant<int, kSrcHbm>;
-    auto two_of = [&](auto S0c, int s1, const Rec &r, const Rec &rn, int nsh, const uint32_t (&cd)[2],
-                      crux_f32x4 (&acc)[NI], int32_t &ex) {
+    auto two_of = [&](auto S0c, int s1, const Rec &r, crux_f32x4 (&acc)[NI], int32_t &ex) {
         if (s1 == kSrcTip)
-            two(S0c, Tip{}, r, rn, nsh, cd, acc, ex);
+            two(S0c, Tip{}, r, acc, ex);
         else if (s1 == kSrcStk)
-            two(S0c, Stk{}, r, rn, nsh, cd, acc, ex);
+            two(S0c, Stk{}, r, acc, ex);
         else if (s1 == kSrcHbm)
-            two(S0c, Hbm{}, r, rn, nsh, cd, acc, ex);
+            two(S0c, Hbm{}, r, acc, ex);
         else if constexpr (decltype(S0c)::value != kSrcFwd)
-            two(S0c, Fwd{}, r, rn, nsh, cd, acc, ex);
+            two(S0c, Fwd{}, r, acc, ex);
     };
 
     Rec r0 = rec(0);
@@ -1776,13 +1773,13 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         if (shape != 0) {
             const int s0 = src_of(shape, 0), s1 = src_of(shape, 1);
             if (s0 == kSrcTip)
-                two_of(Tip{}, s1, r0, r1, nsh, cd, acc, ex);
+                two_of(Tip{}, s1, r0, acc, ex);
             else if (s0 == kSrcFwd)
-                two_of(Fwd{}, s1, r0, r1, nsh, cd, acc, ex);
+                two_of(Fwd{}, s1, r0, acc, ex);
             else if (s0 == kSrcStk)
-                two_of(Stk{}, s1, r0, r1, nsh, cd, acc, ex);
+                two_of(Stk{}, s1, r0, acc, ex);
             else
-                two_of(Hbm{}, s1, r0, r1, nsh, cd, acc, ex);
+                two_of(Hbm{}, s1, r0, acc, ex);
         } else {
             generic(o, r0, cdc, acc, ex);
         }
