@@ -21,6 +21,7 @@ CRUX_TUNE_STACK_DEPTH = 1
 CRUX_TUNE_PRECISION = 2
 CRUX_TUNE_CHUNK = 3
 CRUX_TUNE_WAVES = 4
+CRUX_TUNE_SHORT_PLANS = 5
 CRUX_PREC_F32, CRUX_PREC_BF16, CRUX_PREC_F16, CRUX_PREC_F16X3 = range(4)
 
 # Mixture components (over all models, +I counted) a likelihood may have:
@@ -87,6 +88,8 @@ SIGNATURES = {
     "crux_ctx_last_traffic": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_int)]),
+    "crux_ctx_last_launches": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int)]),
 }
 
 
@@ -260,6 +263,14 @@ class Context:
         check(lib().crux_ctx_last_traffic(self.handle, ctypes.byref(b), ctypes.byref(f),
                                           ctypes.byref(n)))
         return b.value, f.value, n.value
+
+    def last_launches(self):
+        """(traversal launches, of which short-plan kernel launches) of the
+        last batch."""
+        n = ctypes.c_int()
+        k = ctypes.c_int()
+        check(lib().crux_ctx_last_launches(self.handle, ctypes.byref(n), ctypes.byref(k)))
+        return n.value, k.value
 
 
 _contexts = {}

@@ -64,6 +64,16 @@ struct Op {
 
 constexpr int kTipChunk = 64;   // tip mask rows staged in LDS at a time (power of 2)
 
+// Short 4-state plans (partial re-evaluations, MC3 proposals) of at most
+// kPartMaxOps updates, two or four components, run on the short-plan
+// kernel: the whole plan's P images and control words staged in LDS at
+// once, stack depth kPartDepth, tip rows and cached children loaded
+// kPartAhead updates ahead (two: as fast as four or faster at 5k-100k
+// patterns, with fewer registers and loads in flight).
+constexpr int kPartMaxOps = 24;
+constexpr int kPartDepth = 2;
+constexpr int kPartAhead = 2;
+
 __host__ __device__ inline int op_pos(int32_t packed, int j)
 {
     return (int)(int8_t)((uint32_t)packed >> (8 * j));
@@ -200,6 +210,7 @@ struct TravShape {
     int chunk;
     int waves;
     int prec;
+    bool part = false;   // 4 states: the short-plan kernel (depth kPartDepth)
 };
 
 // Traversal classes (K, CM) with a kernel: 4 states at 1-16 components, 20
@@ -262,6 +273,18 @@ enum ChildSrc : int {
 };
 
 constexpr int kCtlWords = 8;
+// Word 3 of the control words: bit 0 has_parent, bits 8-10 the children that
+// are the plan's own results read back from HBM (not loadable ahead), bits
+// 12-14 (short plans) the children loaded ahead that are CLVs cached by an
+// earlier plan, bits 16-18 (short plans) those that are tips.
+constexpr int kCtlOwnShift = 8;
+constexpr int kCtlAheadShift = 12;
+constexpr int kCtlTipShift = 16;
+// Short 4-state plans: kCtlWordsPart words per update, the kCtlWords above
+// and then, in words kCtlAhead + 2 j, what child j loads ahead: its CLV slot
+// or its padded tip row (0: nothing).
+constexpr int kCtlWordsPart = 16;
+constexpr int kCtlAhead = 8;
 constexpr int kCtlWords20 = 32;
 // 20/22-state control words: words 0-3 as for 4 states (flags and shape,
 // pos, tip_load, has_parent), then gfx950 buffer descriptors (4 words each)

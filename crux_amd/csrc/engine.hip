@@ -77,6 +77,7 @@ struct crux_ctx {
     int chunk4 = 0;                // 4-state chunk choice (0 = default, 1-3)
     int waves20 = 0;               // 20/22-state waves per SIMD (0 = default)
     int prec = kPrecF32;           // input precision of 20/22-state products
+    bool short_plans = true;       // 4-state short plans on the short-plan kernel
     // Per-class settings of the three shape keys (-1: the context's value),
     // indexed by class_index(K, CM).
     int ctune[kNumClasses][3];
@@ -91,6 +92,8 @@ struct crux_ctx {
     uint64_t last_bytes = 0;      // HBM bytes the last evaluation had to move
     uint64_t last_formula = 0;    // same with every child CLV read from HBM
     int last_nodes = 0;
+    int last_launches = 0;        // traversal launches of the last batch
+    int last_short = 0;           // ... of which ran the short-plan kernel
     // Two staging sets, so the host plans and uploads one batch while the
     // device runs the previous one (crux_batch_submit / crux_batch_wait).
     // Batches run in submission order on the one stream, which is what makes
@@ -1025,6 +1028,8 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     ctx->last_bytes = 0;
     ctx->last_formula = 0;
     ctx->last_nodes = 0;
+    ctx->last_launches = 0;
+    ctx->last_short = 0;
     if (items.empty()) {
         // Nothing to run: the values are the last ones, once any pending
         // batch has delivered them.
@@ -1051,9 +1056,22 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         cls_tiles[class_index(it.L->tree->K, it.CM)] +=
             (it.L->tree->S + tile_sites(it.L->tree->K, it.CM) - 1) / tile_sites(it.L->tree->K, it.CM);
     TravShape cls_shape[kNumClasses];
+    // A 4-state class whose plans are all short runs the short-plan kernel.
+    bool cls_short[kNumClasses];
+    for (int ci = 0; ci < kNumClasses; ci++)
+        cls_short[ci] = true;
+    for (Item &it : items) {
+        const int ci = class_index(it.L->tree->K, it.CM);
+        if (it.L->tree->K != 4 || (it.CM != 2 && it.CM != 4) || it.L->ops.size() > (size_t)kPartMaxOps)
+            cls_short[ci] = false;
+    }
     for (Item &it : items) {
         const int ci = class_index(it.L->tree->K, it.CM);
         cls_shape[ci] = trav_shape(ctx, it.L->tree->K, it.CM, cls_tiles[ci]);
+        if (cls_short[ci] && ctx->mode != 1 && ctx->short_plans) {
+            cls_shape[ci].part = true;
+            cls_shape[ci].depth = kPartDepth;
+        }
     }
     // Sizes.
     size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0, ntips_total = 0;
@@ -1068,16 +1086,19 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         // launch).
         if (!plain)
             schedule_stack(L->ops, cls_shape[class_index(K, it.CM)].depth);
-        // The 20/22-state traversal runs the plan's last update apart from
-        // the others: only that one may lack a parent (it is the root).
-        if (K != 4)
+        // The 20/22-state and short-plan traversals run the plan's last
+        // update apart from the others: only that one may lack a parent (it
+        // is the root).
+        if (K != 4 || cls_shape[class_index(K, it.CM)].part)
             for (size_t o = 0; o + 1 < L->ops.size(); o++)
                 if (L->ops[o].parent_slot < 0)
                     return set_error(CRUX_EINVAL, "plan has a root update before its end");
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
         nops += L->ops.size();
-        ctl_words += L->ops.size() * (K == 4 ? kCtlWords : kCtlWords20);
+        ctl_words += L->ops.size() * (K != 4 ? kCtlWords20
+                                      : cls_shape[class_index(K, it.CM)].part ? kCtlWordsPart
+                                                                              : kCtlWords);
         if (plain)
             nplain += L->ops.size();
         if (K == 4)
@@ -1202,7 +1223,8 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         const int K = t->K, C = L->C;
         const Arena &A = t->arena[L->pol];
         const int CM = items[ii].CM, KP = kpad(K);
-        const int nw = K == 4 ? kCtlWords : kCtlWords20;
+        const bool part = K == 4 && cls_shape[class_index(K, CM)].part;
+        const int nw = K != 4 ? kCtlWords20 : part ? kCtlWordsPart : kCtlWords;
         const size_t ntl = (t->S + tile_sites(K, CM) - 1) / tile_sites(K, CM);
         const size_t op0 = io, ctl0 = ictl;
         const uint64_t clv_b = (uint64_t)t->S * C * KP * 4, exp_b = (uint64_t)t->exp_count(C) * 4;
@@ -1229,13 +1251,37 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                         ((pop.parent_slot >= 0 ? 1 : 0) << 24) | (op_shape(op, pop) << 25);
                 cw[1] = op.pos;
                 cw[2] = op.tip_load;
-                cw[3] = pop.parent_slot >= 0 ? 1 : 0;
+                // has_parent, and in bits 8+ the children that are this
+                // plan's own results read back from HBM (not loadable ahead).
+                cw[3] = (pop.parent_slot >= 0 ? 1 : 0) | (pop.own << kCtlOwnShift);
                 const uint64_t pp = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.clv(pop.parent_slot) : nullptr);
                 const uint64_t pe = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.exps(pop.parent_slot) : nullptr);
                 cw[4] = (int32_t)(uint32_t)pp;
                 cw[5] = (int32_t)(uint32_t)(pp >> 32);
                 cw[6] = (int32_t)(uint32_t)pe;
                 cw[7] = (int32_t)(uint32_t)(pe >> 32);
+                if (part) {
+                    // Children loaded ahead by the short-plan kernel: tip
+                    // rows, and CLVs cached by an earlier plan (not
+                    // forwarded, not on the stack, not this plan's own
+                    // results).
+                    const int skip = op.from_stack | pop.fwd | pop.own;
+                    for (int j = 0; j < op.nchild; j++) {
+                        const int kind = (op.kinds >> (2 * j)) & 3;
+                        uint64_t pc = 0;
+                        if (kind == kChildTip) {
+                            pc = (uint64_t)(uintptr_t)op.ch[j].codes;
+                            cw[3] |= 1 << (kCtlTipShift + j);
+                        } else if (kind == kChildInternal && !((skip >> j) & 1)) {
+                            pc = (uint64_t)(uintptr_t)op.ch[j].clv;
+                            cw[3] |= 1 << (kCtlAheadShift + j);
+                        } else {
+                            continue;
+                        }
+                        cw[kCtlAhead + 2 * j] = (int32_t)(uint32_t)pc;
+                        cw[kCtlAhead + 2 * j + 1] = (int32_t)(uint32_t)(pc >> 32);
+                    }
+                }
                 if (K == 4)
                     return;
                 // 20/22 states: buffer descriptors the kernel loads into
@@ -1401,10 +1447,13 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             first = std::min(first, i);
             last = i;
         }
+    int nlaunch = 0, nshort = 0;
     for (size_t i = 0; i < launches.size(); i++) {
         const Launch &ln = launches[i];
         if (ln.nblk == 0)
             continue;
+        nlaunch++;
+        nshort += cls_shape[class_index(ln.K, ln.CM)].part ? 1 : 0;
         if (launch_traverse(ln.K, ln.CM, dent, dblk, (int)ln.blk0, (int)ln.nblk,
                             cls_shape[class_index(ln.K, ln.CM)], st,
                             tm && i == first ? S.ev[2] : nullptr,
@@ -1424,6 +1473,8 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     ctx->last_bytes = traffic;
     ctx->last_formula = formula;
     ctx->last_nodes = nodes;
+    ctx->last_launches = nlaunch;
+    ctx->last_short = nshort;
     S.pending = true;
     S.timed = ctx->timing;
     S.ticket = ctx->next_ticket++;
@@ -1676,6 +1727,12 @@ int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value)
         ctx->prec = value;
         return CRUX_OK;
     }
+    if (key == CRUX_TUNE_SHORT_PLANS) {
+        if (value != 0 && value != 1)
+            return set_error(CRUX_EINVAL, "short plans %d (0 or 1)", value);
+        ctx->short_plans = value != 0;
+        return CRUX_OK;
+    }
     if (int rc = check_shape_value(key, value))
         return rc;
     *ctx_shape_field(ctx, key) = value;
@@ -1688,6 +1745,10 @@ int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value)
         return set_error(CRUX_EINVAL, "NULL argument");
     if (key == CRUX_TUNE_PRECISION) {
         *value = ctx->prec;
+        return CRUX_OK;
+    }
+    if (key == CRUX_TUNE_SHORT_PLANS) {
+        *value = ctx->short_plans ? 1 : 0;
         return CRUX_OK;
     }
     if (key != CRUX_TUNE_STACK_DEPTH && key != CRUX_TUNE_CHUNK && key != CRUX_TUNE_WAVES)
@@ -1788,6 +1849,17 @@ int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, uint64_t *formul
         *formula_bytes = ctx->last_formula;
     if (nodes)
         *nodes = ctx->last_nodes;
+    return CRUX_OK;
+}
+
+int crux_ctx_last_launches(const crux_ctx *ctx, int *launches, int *short_launches)
+{
+    if (!ctx)
+        return set_error(CRUX_EINVAL, "ctx is NULL");
+    if (launches)
+        *launches = ctx->last_launches;
+    if (short_launches)
+        *short_launches = ctx->last_short;
     return CRUX_OK;
 }
 

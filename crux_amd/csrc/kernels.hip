@@ -274,6 +274,9 @@ __device__ __forceinline__ int32_t bload1(__amdgpu_buffer_rsrc_t r, uint32_t vof
 // s_waitcnt immediate that waits for LDS and scalar loads only (vmcnt and
 // expcnt at their maxima, lgkmcnt 0).
 constexpr int kWaitLgkm0 = 0xC07F;
+// ... and one that waits for vector memory only (vmcnt 0, expcnt and lgkmcnt
+// at their maxima).
+constexpr int kWaitVm0 = 0x0F70;
 
 // Cache policy of a store: 0, or kNT (the non-temporal bit of gfx950's
 // buffer instructions) for CLV stores, which a later evaluation reads back,
@@ -1225,6 +1228,383 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
 }
 
 // ---------------------------------------------------------------------------
+// Short 4-state plans (partial re-evaluations, MC3 proposals: at most
+// kPartMaxOps updates), two or four components.  The lanes, slots,
+// arithmetic and results are k_trav4h's, bit for bit; what differs is where
+// the inputs come from.  In a short plan almost every update takes a child
+// cached in HBM by an earlier plan, and k_trav4h's one-update-ahead load of
+// it leaves each update waiting for most of an HBM round trip.  Here:
+// * the P images and control words of every update are staged into LDS
+//   once, every load issued before the first wait, behind one barrier;
+// * the children of update o + R that come from memory (tip rows, and CLVs
+//   cached by an earlier plan: the host lists them, with their addresses, in
+//   the control words) are loaded while update o runs, into a ring of R
+//   register slots.  The update loop is unrolled R times so that each slot is
+//   a fixed set of registers, and every update issues the same loads and
+//   stores (a child with nothing to load reads an empty range), so that the
+//   compiler's wait for a slot counts exactly the updates issued since.  The
+//   first R updates' loads go out with the staging loads, from control words
+//   read with scalar loads, and the plan's root takes its slot like any
+//   other update;
+// * the plan's own results re-read from HBM (after the stack overflowed) and
+//   identity children are loaded where they are used, as in k_trav4h's
+//   generic walk.
+// The last update (the plan's root; the host checks that no other update
+// lacks a parent) runs after the loop.
+template <int CM, int D, int R>
+__global__ void __launch_bounds__(2 * kTile4)
+k_part4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_entry, int blk0)
+{
+    static_assert(CM >= 2, "at least one component per lane half");
+    constexpr int CL = CM / 2;                    // components per lane
+    constexpr int HS = kTile4 / 2;                // sites per wave
+    constexpr int NT = 2 * kTile4;                // threads of a block
+    constexpr int NC = kOpMaxChild;
+    constexpr int NR = p4_regs(CM);               // P image registers per update
+    constexpr int kRows = kPartMaxOps * NR * 16;  // float4s of P of a whole plan
+    constexpr int kWords = kPartMaxOps * kCtlWordsPart;
+    static_assert(kRows % NT == 0 && kWords % NT == 0, "staging: a whole number of loads per thread");
+    __shared__ float4 stk[2][D][CL][kTile4];
+    __shared__ int32_t stke[2][D][kTile4];
+    __shared__ float4 pch[kRows];
+    __shared__ int32_t cch[kWords];
+    __shared__ double red[2];
+    const int b = blk0 + (int)blockIdx.x;
+    const CRUX_CAS Entry *E = cas(entries) + cas(block_entry)[b];
+    const int S = E->S, C = E->C, nops = E->nops;
+    const int tile = b - E->tile0;
+    const int tid = (int)threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int h = lane >> 5;
+    const int s = tile * kTile4 + wv * HS + (lane & 31);
+    const bool act = s < S;
+    const int sc = act ? s : S - 1;       // in-bounds site for loads of idle lanes
+    const CRUX_CAS Op *ops = cas(E->ops);
+    const uint32_t exp_off = (uint32_t)E->exp_off;
+    // Bytes of a padded tip row (a multiple of the tile).
+    const uint32_t row_bytes = (uint32_t)((S + kTile4 - 1) / kTile4 * kTile4);
+    double sum = 0.0;
+    // Slot offsets as in k_trav4h (kOut: outside every range).
+    constexpr uint32_t kOut = 0xfffffff0u;
+    uint32_t vo[CL], vol[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+        const bool comp = h == 0 || CL + m < C;
+        const auto at = [&](int site) {
+            return (uint32_t)((((site >> 5) * CL + m) * 2 + h) * 32 + (site & 31)) * 16;
+        };
+        vo[m] = act && comp ? at(s) : kOut;
+        vol[m] = comp ? at(sc) : kOut;
+    }
+    const uint32_t voe = (act && h == 0) ? (uint32_t)s * 4 : kOut;
+    // Offsets of this lane's exponent, and of the dword of its tip mask byte.
+    const uint32_t voex = (uint32_t)sc * 4, votip = (uint32_t)sc & ~3u;
+
+    auto lds_barrier = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+
+    // What update o's children load ahead, into a ring slot: CL float4 and a
+    // dword each (a cached CLV and its exponent, or the dword of a tip row
+    // holding this lane's mask byte), the same loads whatever the children
+    // are (an empty range for a child with nothing to load, or o past the
+    // plan's end).  cw0: the plan's control words, in LDS or in memory.
+    auto load_ahead = [&](int o, const auto *cw0, float4 (&x)[NC][CL], int32_t (&e)[NC]) {
+        const bool in = o < nops;
+        const auto *cw = cw0 + (in ? o : 0) * kCtlWordsPart;
+        const uint32_t w3 = in ? (uint32_t)__builtin_amdgcn_readfirstlane(cw[3]) : 0u;
+        static_for<NC>([&](auto Jc) {
+            constexpr int j = decltype(Jc)::value;
+            const bool cl = (w3 >> (kCtlAheadShift + j)) & 1, tp = (w3 >> (kCtlTipShift + j)) & 1;
+            const uint64_t p = (uint32_t)__builtin_amdgcn_readfirstlane(cw[kCtlAhead + 2 * j]) |
+                               ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[kCtlAhead + 2 * j + 1]) << 32);
+            const auto rc = rsrc_of(reinterpret_cast<const void *>(p), cl ? exp_off : 0u);
+#pragma unroll
+            for (int m = 0; m < CL; ++m) {
+                const crux_f32x4 v = bload4(rc, vol[m], 0);
+                x[j][m] = make_float4(v.x, v.y, v.z, v.w);
+            }
+            const auto re = rsrc_of(reinterpret_cast<const void *>(cl ? p + exp_off : p),
+                                    cl ? (uint32_t)S * 4 : tp ? row_bytes : 0u);
+            e[j] = bload1(re, tp ? votip : voex, 0);
+        });
+    };
+
+    // Staging, with the first R updates' ring loads: every load issued, then
+    // the staged data written to LDS behind one barrier (the ring loads stay
+    // outstanding).
+    float4 hx[R][NC][CL];
+    int32_t hxe[R][NC];
+    {
+        const int nr = nops * NR * 16, nc = nops * kCtlWordsPart;
+        const auto rp = rsrc_of(E->pdense, (uint32_t)nr * 16);
+        const auto rw = rsrc_of(E->ctl, (uint32_t)nc * 4);
+        crux_f32x4 pv[kRows / NT];
+        int32_t cv[kWords / NT];
+#pragma unroll
+        for (int k = 0; k < kRows / NT; ++k)
+            pv[k] = bload4(rp, (uint32_t)(tid + k * NT) * 16, 0);
+#pragma unroll
+        for (int k = 0; k < kWords / NT; ++k)
+            cv[k] = bload1(rw, (uint32_t)(tid + k * NT) * 4, 0);
+        static_for<R>([&](auto Kc) {
+            constexpr int k = decltype(Kc)::value;
+            load_ahead(k, cas(E->ctl), hx[k], hxe[k]);
+        });
+#pragma unroll
+        for (int k = 0; k < kRows / NT; ++k)
+            pch[tid + k * NT] = make_float4(pv[k].x, pv[k].y, pv[k].z, pv[k].w);
+#pragma unroll
+        for (int k = 0; k < kWords / NT; ++k)
+            cch[tid + k * NT] = cv[k];
+        lds_barrier();
+    }
+
+    float4 yl[CL];
+    int32_t el = 0;
+#pragma unroll
+    for (int m = 0; m < CL; ++m)
+        yl[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    // Update o with what its children loaded ahead in hx/hxe (or loaded here:
+    // own results, identity children).  Returns the product of the children
+    // and the summed exponent.
+    auto body = [&](int o, const float4 (&hx)[NC][CL], const int32_t (&hxe)[NC], float4 (&acc)[CL],
+                    int32_t &ex, int &pk_out, int &w0_out) {
+        const int32_t *cw = &cch[o * kCtlWordsPart];
+        const int w0 = __builtin_amdgcn_readfirstlane(cw[0]);
+        const int32_t pk = __builtin_amdgcn_readfirstlane(cw[1]);
+        const int w3 = __builtin_amdgcn_readfirstlane(cw[3]);
+        const int nch = w0 & 0xff, kinds = (w0 >> 8) & 0xff, fst = (w0 >> 16) & 0xf,
+                  fwd = (w0 >> 20) & 0xf, ahead = w3 >> kCtlAheadShift;
+        pk_out = pk;
+        w0_out = w0;
+        float pr[NR];
+        {
+            const float *img = reinterpret_cast<const float *>(&pch[o * NR * 16]);
+#pragma unroll
+            for (int v = 0; v < NR; ++v)
+                pr[v] = img[v * 64 + lane];
+        }
+        const CRUX_CAS Op *op = ops + o;
+        ex = 0;
+        static_for<NC>([&](auto Jc) {
+            constexpr int j = decltype(Jc)::value;
+            if (j >= nch)
+                return;
+            const int kind = (kinds >> (2 * j)) & 3;
+            float4 x[CL];
+            if (kind == kChildTip) {
+                const uint32_t mk = (uint32_t)hxe[j] >> (8 * (lane & 3));
+                const float4 t = make_float4((float)(mk & 1u), (float)((mk >> 1) & 1u),
+                                             (float)((mk >> 2) & 1u), (float)((mk >> 3) & 1u));
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    x[m] = t;
+            } else if ((fwd >> j) & 1) {
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    x[m] = yl[m];
+                ex += el;
+            } else if ((fst >> j) & 1) {
+                const int p = op_pos(pk, j) & (D - 1);
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    x[m] = stk[wv][p][m][lane];
+                ex += stke[wv][p][lane];
+            } else if ((ahead >> j) & 1) {
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    x[m] = hx[j][m];
+                ex += hxe[j];
+            } else {
+                const CRUX_CAS OpChild &ch = op->ch[j];
+                const auto rc = rsrc_of(reinterpret_cast<const char *>(ch.clv), exp_off);
+#pragma unroll
+                for (int m = 0; m < CL; ++m) {
+                    const crux_f32x4 v = bload4(rc, vol[m], 0);
+                    x[m] = make_float4(v.x, v.y, v.z, v.w);
+                }
+                ex += bload1(rsrc_of(ch.exps), voex, 0);
+                // Waited for here, so that where this branch joins the ring
+                // slot's, only the slot's loads are outstanding and the wait
+                // the compiler puts at the join counts them, not these.
+                __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            }
+            float4 v[CL];
+            if (kind != kChildIdent) {
+                crux_f32x4 d[CL];
+                static_for<4>([&](auto Kc) {
+                    constexpr int k = decltype(Kc)::value;
+                    static_for<CL>([&](auto Mc) {
+                        constexpr int m = decltype(Mc)::value;
+                        constexpr int q = (j * CL + m) * 4 + k;
+                        const float4 xv = x[m];
+                        const float xk = k == 0 ? xv.x : k == 1 ? xv.y : k == 2 ? xv.z : xv.w;
+                        d[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(
+                            pr[q >> 3], xk, k == 0 ? crux_f32x4{0.f, 0.f, 0.f, 0.f} : d[m], 3, q & 7, 0);
+                    });
+                });
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    v[m] = make_float4(d[m].x, d[m].y, d[m].z, d[m].w);
+            } else {
+#pragma unroll
+                for (int m = 0; m < CL; ++m)
+                    v[m] = x[m];
+            }
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                acc[m] = (j == 0) ? v[m] : mul4(acc[m], v[m]);
+        });
+    };
+    // Rescale (as k_trav4h), store and push.
+    auto finish = [&](int o, const float4 (&acc)[CL], int32_t ex, int32_t pk) {
+        int32_t mb = 0;
+#pragma unroll
+        for (int m = 0; m < CL; ++m)
+            mb = max(max(mb, max(__float_as_int(acc[m].x), __float_as_int(acc[m].y))),
+                     max(__float_as_int(acc[m].z), __float_as_int(acc[m].w)));
+        {
+            uint32_t a0, a1;
+            lane_pair<32>((uint32_t)mb, a0, a1);
+            mb = max((int32_t)a0, (int32_t)a1);
+        }
+        const float mx = __int_as_float(mb);
+        int e2 = 0;
+        if (mx > 0.0f)
+            (void)frexpf(mx, &e2);
+        el = ex + e2;
+#pragma unroll
+        for (int m = 0; m < CL; ++m)
+            yl[m] = make_float4(ldexpf(acc[m].x, -e2), ldexpf(acc[m].y, -e2), ldexpf(acc[m].z, -e2),
+                                ldexpf(acc[m].w, -e2));
+        const int32_t *cw = &cch[o * kCtlWordsPart];
+        const uint64_t par = (uint32_t)__builtin_amdgcn_readfirstlane(cw[4]) |
+                             ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[5]) << 32);
+        const int pp = op_pos(pk, kOpMaxChild);
+        if (pp >= 0) {
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                stk[wv][pp][m][lane] = yl[m];
+            stke[wv][pp][lane] = el;
+        }
+        const auto rp = rsrc_of(reinterpret_cast<const char *>(par), exp_off);
+#pragma unroll
+        for (int m = 0; m < CL; ++m)
+            bstore4<kNT>(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
+        bstore1<kNT>(el, rsrc_of(reinterpret_cast<const char *>(par) + exp_off, (uint32_t)S * 4), voe, 0);
+    };
+
+    // Update o with ring slot k = o % R: its children, then the loads of
+    // update o + R into the slot it freed, then its result.
+    auto step = [&](int o, auto Kc) {
+        constexpr int k = decltype(Kc)::value;
+        float4 acc[CL];
+        int32_t ex = 0;
+        int pk = 0, w0 = 0;
+        body(o, hx[k], hxe[k], acc, ex, pk, w0);
+        load_ahead(o + R, cch, hx[k], hxe[k]);
+        finish(o, acc, ex, pk);
+    };
+    const int nin = nops - 1;
+    int o = 0;
+    for (; o + R <= nin; o += R)
+        static_for<R>([&](auto Kc) { step(o + decltype(Kc)::value, Kc); });
+    static_for<R>([&](auto Kc) {
+        if (o + decltype(Kc)::value < nin)
+            step(o + decltype(Kc)::value, Kc);
+    });
+
+    // The last update, from its ring slot nin - o: the root reduction.
+    {
+        const int ol = nin;
+        float4 hz[NC][CL];
+        int32_t hze[NC];
+        static_for<R>([&](auto Kc) {
+            constexpr int k = decltype(Kc)::value;
+            if (k == ol - o) {
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+#pragma unroll
+                    for (int m = 0; m < CL; ++m)
+                        hz[j][m] = hx[k][j][m];
+                    hze[j] = hxe[k][j];
+                }
+            }
+        });
+        float4 acc[CL];
+        int32_t ex = 0;
+        int pk = 0, w0 = 0;
+        body(ol, hz, hze, acc, ex, pk, w0);
+        if ((w0 >> 24) & 1) {
+            finish(ol, acc, ex, pk);
+        } else {
+            double L = 0.0;
+#pragma unroll
+            for (int m = 0; m < CL; ++m) {
+                const CRUX_CAS double *w0p = cas(E->piw) + 4 * m;
+                double wx = w0p[0], wy = w0p[1], wz = w0p[2], wq = w0p[3];
+                if (CL + m < C) {
+                    const CRUX_CAS double *w1p = cas(E->piw) + 4 * (CL + m);
+                    if (h) {
+                        wx = w1p[0];
+                        wy = w1p[1];
+                        wz = w1p[2];
+                        wq = w1p[3];
+                    }
+                } else if (h) {
+                    wx = wy = wz = wq = 0.0;
+                }
+                L += wx * (double)acc[m].x + wy * (double)acc[m].y + wz * (double)acc[m].z +
+                     wq * (double)acc[m].w;
+            }
+            {
+                const uint64_t u = (uint64_t)__double_as_longlong(L);
+                uint32_t al, bl, ah, bh;
+                lane_pair<32>((uint32_t)u, al, bl);
+                lane_pair<32>((uint32_t)(u >> 32), ah, bh);
+                L = __longlong_as_double((long long)(((uint64_t)ah << 32) | al)) +
+                    __longlong_as_double((long long)(((uint64_t)bh << 32) | bl));
+            }
+            double v = (log(L) + (double)ex * 0.69314718055994530942) * (double)gas(E->weights)[sc];
+            if (__builtin_isnan(v))
+                v = -INFINITY;
+            if (act && h == 0) {
+                gas(E->site_out)[s] = v;
+                sum = v;
+            }
+        }
+    }
+    double cmp = 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double s2 = __shfl_xor(sum, off, 64), c2 = __shfl_xor(cmp, off, 64);
+        if ((lane & off) == 0) {
+            nsum_merge(sum, cmp, s2, c2);
+        } else {
+            double a = s2, bb = c2;
+            nsum_merge(a, bb, sum, cmp);
+            sum = a;
+            cmp = bb;
+        }
+    }
+    if (wv == 1 && lane == 0) {
+        red[0] = sum;
+        red[1] = cmp;
+    }
+    lds_barrier();
+    if (wv == 0 && lane == 0) {
+        nsum_merge(sum, cmp, red[0], red[1]);
+        gas(E->partials)[2 * tile] = sum;
+        gas(E->partials)[2 * tile + 1] = cmp;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // 20/22-state traversal.  A block is one wave covering CM mixture components
 // of G = 64 / CM consecutive sites: lane l holds component l / G of site
 // l % G, all KP states of it in registers (states >= K stay 0).  Like the
@@ -1813,7 +2193,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         for (int j = 0; j < kOpMaxChild; ++j)
             cdc[j] = code_of(r0.pk(), j);
         generic(nops - 1, r0, cdc, acc, ex);
-        if (r0.hp() != 0) {
+        if ((r0.hp() & 1) != 0) {
             rescale(acc, ex);
             store(mk(r0.par()), r0.pk());
         } else {
@@ -1924,6 +2304,13 @@ void launch_class(const Entry *entries, const int32_t *block_entry, int blk0, in
     // Stack depth: more slots keep more results in LDS, fewer let more
     // blocks share a CU.
     if constexpr (K == 4) {
+        if constexpr (CM == 2 || CM == 4) {
+            if (sh.part) {
+                hipExtLaunchKernelGGL((k_part4h<CM, kPartDepth, kPartAhead>), dim3(nblocks), dim3(2 * kTile4),
+                                      0, stream, t0, t1, 0, entries, block_entry, blk0);
+                return;
+            }
+        }
         if (sh.depth <= 2)
             launch_trav4_chunk<CM, 2>(entries, block_entry, blk0, nblocks, sh.chunk, stream, t0, t1);
         else if (sh.depth <= 4)

@@ -272,10 +272,22 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  *   or CRUX_PREC_F16X3 (float16 high and low parts, three products).  The
  *   16-bit forms run at stack depth 4 and the default waves.  Likelihoods
  *   evaluated at another precision recompute their CLVs.
+ *   CRUX_TUNE_SHORT_PLANS: 1 (default) runs 4-state classes whose plans are
+ *   all short (at most 24 updates, two or four components: partial
+ *   re-evaluations, MC3 proposals) on the short-plan kernel, which stages
+ *   the whole plan at once and loads tip rows and cached children two
+ *   updates ahead; 0 runs them on the general kernel.  Same results, bit
+ *   for bit, either way.
  * Settings apply to batches submitted after the call; do not change them
  * while another thread submits on the context. */
 enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
-enum { CRUX_TUNE_STACK_DEPTH = 1, CRUX_TUNE_PRECISION = 2, CRUX_TUNE_CHUNK = 3, CRUX_TUNE_WAVES = 4 };
+enum {
+    CRUX_TUNE_STACK_DEPTH = 1,
+    CRUX_TUNE_PRECISION = 2,
+    CRUX_TUNE_CHUNK = 3,
+    CRUX_TUNE_WAVES = 4,
+    CRUX_TUNE_SHORT_PLANS = 5
+};
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
 
@@ -318,6 +330,14 @@ int crux_ctx_last_host_timing(const crux_ctx *ctx, double *us4);
  * updates. */
 int crux_ctx_last_traffic(const crux_ctx *ctx, uint64_t *bytes, uint64_t *formula_bytes,
                           int *nodes);
+
+/* Traversal kernel launches of the last batch (one per state/component
+ * class with work; one per node update in one-launch-per-update mode), and
+ * how many of them ran the short-plan kernel (4 states, two or four
+ * components, every plan of the class at most 24 node updates; see
+ * CRUX_TUNE_SHORT_PLANS).  A measurement hook with no counterpart in the
+ * reference. */
+int crux_ctx_last_launches(const crux_ctx *ctx, int *launches, int *short_launches);
 
 #ifdef __cplusplus
 }
