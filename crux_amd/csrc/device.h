@@ -281,8 +281,8 @@ constexpr int kCtlOwnShift = 8;
 constexpr int kCtlAheadShift = 12;
 constexpr int kCtlTipShift = 16;
 // Short 4-state plans: kCtlWordsPart words per update, the kCtlWords above
-// and then, in words kCtlAhead + 2 j, what child j loads ahead: its CLV slot
-// or its padded tip row (0: nothing).
+// and then, in words kCtlAhead + 2 j, child j's CLV slot or padded tip row
+// (bits 12-18 of word 3 say which of them are loaded ahead).
 constexpr int kCtlWordsPart = 16;
 constexpr int kCtlAhead = 8;
 constexpr int kCtlWords20 = 32;

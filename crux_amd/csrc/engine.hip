@@ -1261,10 +1261,11 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                 cw[6] = (int32_t)(uint32_t)pe;
                 cw[7] = (int32_t)(uint32_t)(pe >> 32);
                 if (part) {
-                    // Children loaded ahead by the short-plan kernel: tip
-                    // rows, and CLVs cached by an earlier plan (not
-                    // forwarded, not on the stack, not this plan's own
-                    // results).
+                    // Every child's slot or tip row, and which of them the
+                    // short-plan kernel loads ahead: tip rows, and CLVs
+                    // cached by an earlier plan (not forwarded, not on the
+                    // stack, not this plan's own results).  A child read
+                    // where it is used takes its slot from these words too.
                     const int skip = op.from_stack | pop.fwd | pop.own;
                     for (int j = 0; j < op.nchild; j++) {
                         const int kind = (op.kinds >> (2 * j)) & 3;
@@ -1272,11 +1273,10 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                         if (kind == kChildTip) {
                             pc = (uint64_t)(uintptr_t)op.ch[j].codes;
                             cw[3] |= 1 << (kCtlTipShift + j);
-                        } else if (kind == kChildInternal && !((skip >> j) & 1)) {
-                            pc = (uint64_t)(uintptr_t)op.ch[j].clv;
-                            cw[3] |= 1 << (kCtlAheadShift + j);
                         } else {
-                            continue;
+                            pc = (uint64_t)(uintptr_t)op.ch[j].clv;
+                            if (kind == kChildInternal && !((skip >> j) & 1))
+                                cw[3] |= 1 << (kCtlAheadShift + j);
                         }
                         cw[kCtlAhead + 2 * j] = (int32_t)(uint32_t)pc;
                         cw[kCtlAhead + 2 * j + 1] = (int32_t)(uint32_t)(pc >> 32);

@@ -1280,7 +1280,6 @@ k_part4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     const int s = tile * kTile4 + wv * HS + (lane & 31);
     const bool act = s < S;
     const int sc = act ? s : S - 1;       // in-bounds site for loads of idle lanes
-    const CRUX_CAS Op *ops = cas(E->ops);
     const uint32_t exp_off = (uint32_t)E->exp_off;
     // Bytes of a padded tip row (a multiple of the tile).
     const uint32_t row_bytes = (uint32_t)((S + kTile4 - 1) / kTile4 * kTile4);
@@ -1389,7 +1388,6 @@ k_part4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             for (int v = 0; v < NR; ++v)
                 pr[v] = img[v * 64 + lane];
         }
-        const CRUX_CAS Op *op = ops + o;
         ex = 0;
         static_for<NC>([&](auto Jc) {
             constexpr int j = decltype(Jc)::value;
@@ -1421,14 +1419,17 @@ k_part4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                     x[m] = hx[j][m];
                 ex += hxe[j];
             } else {
-                const CRUX_CAS OpChild &ch = op->ch[j];
-                const auto rc = rsrc_of(reinterpret_cast<const char *>(ch.clv), exp_off);
+                // The child's slot, from the same words as loads ahead use.
+                const uint64_t p = (uint32_t)__builtin_amdgcn_readfirstlane(cw[kCtlAhead + 2 * j]) |
+                                   ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(cw[kCtlAhead + 2 * j + 1])
+                                    << 32);
+                const auto rc = rsrc_of(reinterpret_cast<const void *>(p), exp_off);
 #pragma unroll
                 for (int m = 0; m < CL; ++m) {
                     const crux_f32x4 v = bload4(rc, vol[m], 0);
                     x[m] = make_float4(v.x, v.y, v.z, v.w);
                 }
-                ex += bload1(rsrc_of(ch.exps), voex, 0);
+                ex += bload1(rsrc_of(reinterpret_cast<const void *>(p + exp_off), (uint32_t)S * 4), voex, 0);
                 // Waited for here, so that where this branch joins the ring
                 // slot's, only the slot's loads are outstanding and the wait
                 // the compiler puts at the join counts them, not these.

@@ -52,8 +52,9 @@ def _moves(lik, rng, n, multi):
 
 
 def _run(lik, moves, mode, oracle=None):
-    """Site lnLs of the base evaluation and after each move, with the
-    short-plan kernel on (mode 1) or off; the tree is restored after."""
+    """Site lnLs of the base evaluation and after each move under short-plan
+    mode `mode`, and the launches that ran the short-plan kernel; the tree
+    is restored after."""
     ctx = nat.get_context(0)
     ctx.set_tuning(nat.CRUX_TUNE_SHORT_PLANS, mode)
     edges = list(lik.tree.edges)
