@@ -906,7 +906,11 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     const int buf = k & 1, o0 = k * PC, n = min(PC, nops - o0);
     if (k > 0 && k + 1 < nchunks)
         fetch(k + 1);
-    for (int oq = 0; oq < n; ++oq) {
+    // At least one update per chunk, and every update issues its stores (the
+    // root's to an empty range): the compiler's wait for the fetch at the
+    // chunk's end then never counts the last update's stores.
+    int oq = 0;
+    do {
         const CRUX_CAS Op *op = ops + o0 + oq;
         const int w0 = __builtin_amdgcn_readfirstlane(cwa.x);
         const int32_t pk = __builtin_amdgcn_readfirstlane(cwa.y);
@@ -1140,6 +1144,15 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         for (int m = 0; m < CL; ++m)
             yl[m] = make_float4(ldexpf(acc[m].x, -e2), ldexpf(acc[m].y, -e2),
                                 ldexpf(acc[m].z, -e2), ldexpf(acc[m].w, -e2));
+        {
+            const char *pp0 = reinterpret_cast<const char *>(par);
+            const auto rp = rsrc_of(pp0, has_parent ? (uint32_t)E->exp_off : 0u);
+#pragma unroll
+            for (int m = 0; m < CL; ++m)
+                bstore4<kNT>(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
+            bstore1<kNT>(el, rsrc_of(reinterpret_cast<const void *>(pex), has_parent ? (uint32_t)S * 4 : 0u),
+                         voe, 0);
+        }
         if (has_parent) {
             const int pp = op_pos(pk, kOpMaxChild);
             if (pp >= 0) {
@@ -1148,12 +1161,6 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                     stk[wv][pp][m][lane] = yl[m];
                 stke[wv][pp][lane] = el;
             }
-            const char *pp0 = reinterpret_cast<const char *>(par);
-            const auto rp = rsrc_of(pp0, (uint32_t)E->exp_off);
-#pragma unroll
-            for (int m = 0; m < CL; ++m)
-                bstore4<kNT>(crux_f32x4{yl[m].x, yl[m].y, yl[m].z, yl[m].w}, rp, vo[m], 0);
-            bstore1<kNT>(el, rsrc_of(reinterpret_cast<const void *>(pex), (uint32_t)S * 4), voe, 0);
         } else {
             double L = 0.0;
 #pragma unroll
@@ -1192,7 +1199,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
                 sum = v;
             }
         }
-    }
+    } while (++oq < n);
     // The next chunk goes into the other buffer; its first update is read
     // after the barrier.
     if (k + 1 < nchunks) {
