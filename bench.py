@@ -152,17 +152,21 @@ def bench_dna(args, out):
         lik.flush()
         return nat.batch_submit(ctx, [lik], [lik._ready(root)])
 
-    pending = submit_full()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        nxt = submit_full()
+    pipe_eps = None
+    try:
+        pending = submit_full()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            nxt = submit_full()
+            lnl_p = float(nat.batch_wait(ctx, pending)[0])
+            pending = nxt
         lnl_p = float(nat.batch_wait(ctx, pending)[0])
-        pending = nxt
-    lnl_p = float(nat.batch_wait(ctx, pending)[0])
-    dt = time.perf_counter() - t
-    pipe_eps = args.steps / dt
-    log("dna full, two in flight: %.1f evals/s (lnL %s the synchronous one)"
-        % (pipe_eps, "equals" if lnl_p == lnl0 else "DIFFERS FROM"))
+        dt = time.perf_counter() - t
+        pipe_eps = args.steps / dt
+        log("dna full, two in flight: %.1f evals/s (lnL %s the synchronous one)"
+            % (pipe_eps, "equals" if lnl_p == lnl0 else "DIFFERS FROM"))
+    except Exception as e:  # the headline figures do not depend on it
+        log("dna full, two in flight: failed (%s)" % e)
 
     # Traversal kernel duration: HIP events the runtime stamps at the
     # kernel's start and end on the context's stream, averaged over as many
@@ -249,7 +253,7 @@ def bench_dna(args, out):
         "config": {"workload": "single large DNA likelihood: 128 taxa, 100,000 sites -> "
                                "%d patterns, GTR+G4 alpha=0.5, full traversal" % S,
                    "patterns": S, "taxa": 128, "categories": 4},
-        "full_evals_per_s_two_in_flight": round(pipe_eps, 2),
+        "full_evals_per_s_two_in_flight": None if pipe_eps is None else round(pipe_eps, 2),
         "partial_evals_per_s": round(part_eps, 2),
         "partial_node_updates": round(float(np.mean(part_nodes)), 2),
         "plain_gpu_evals_per_s": round(plain_eps, 2),
