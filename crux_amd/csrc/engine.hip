@@ -315,6 +315,14 @@ struct crux_tree {
     std::vector<int32_t> ring_node, ring_next, node_ring, node_taxon, degree;
     std::vector<double> edge_len;
     std::vector<RingCache> cache[2];
+    // Per polarity and ring: nothing behind the ring (its node's side of the
+    // tree away from its edge) changed since that polarity's planner last
+    // walked it and left every CL there valid and keyed.  Closed under
+    // subtrees: a clean ring's whole subtree is clean.  The planner skips a
+    // clean subtree with a valid CL at its top (checking only that ring's
+    // own key); a length edit clears the rings whose subtree holds the edge,
+    // a topology edit clears all.
+    std::vector<uint8_t> clean[2];
     Arena arena[2];
     crux_lik *owner[2] = {nullptr, nullptr};
     std::vector<crux_lik *> liks;   // every likelihood created on the tree
@@ -401,6 +409,37 @@ static void release_ring(crux_tree *t, int32_t r)
             arena_put(t, p, t->cache[p][r].slot);
             t->cache[p][r] = RingCache{};
         }
+}
+
+// A length edit of edge e: clear the rings whose subtree holds e, i.e. on
+// each side of e every ring of every node reached except the one the walk
+// came in by.  Subtrees grow away from e and clean rings are closed under
+// subtrees, so the walk stops at a ring that is already unclean.
+static void mark_edit(crux_tree *t, int32_t e)
+{
+    thread_local std::vector<int32_t> todo;
+    todo.clear();
+    todo.push_back(2 * e);
+    todo.push_back(2 * e + 1);
+    while (!todo.empty()) {
+        const int32_t in = todo.back();
+        todo.pop_back();
+        if (t->ring_node[in] < 0)
+            continue;
+        for (int32_t r = t->ring_next[in]; r != in; r = t->ring_next[r]) {
+            if (!t->clean[0][r] && !t->clean[1][r])
+                continue;
+            t->clean[0][r] = 0;
+            t->clean[1][r] = 0;
+            todo.push_back(r ^ 1);
+        }
+    }
+}
+
+static void mark_all(crux_tree *t)
+{
+    for (int p = 0; p < 2; p++)
+        std::fill(t->clean[p].begin(), t->clean[p].end(), (uint8_t)0);
 }
 
 static int arena_layout(crux_tree *t, int pol, int C)
@@ -528,6 +567,18 @@ struct Planner {
         const int node = t->ring_node[ring];
         const int deg = t->degree[node];
         RingCache &me = t->cache[pol][ring];
+        if (!inval && t->clean[pol][ring] && (deg == 1 || (me.valid && me.slot >= 0))) {
+            // Nothing behind this ring changed since its subtree was last
+            // walked: the walk would find every key there equal and every CL
+            // valid.  Only this ring's own key can differ.
+            if (me.parent != parent_key || me.nsibs != nsibs || me.elen != elen) {
+                valid_of(parent_key) = false;
+                me.parent = parent_key;
+                me.nsibs = nsibs;
+                me.elen = elen;
+            }
+            return;
+        }
         if (deg == 1) {
             if (t->node_taxon[node] == -1) {
                 rc = set_error(CRUX_ETAXON, "Leaf node missing taxon");
@@ -571,6 +622,8 @@ struct Planner {
             me.valid = true;
             valid_of(parent_key) = false;
         }
+        if (!rc)
+            t->clean[pol][ring] = 1;
     }
 
     void plan(int root)
@@ -2008,6 +2061,8 @@ int crux_tree_set_topology(crux_tree *t, int nnodes, int nedges, const int32_t *
     t->node_taxon.assign(node_taxon, node_taxon + nnodes);
     t->edge_len.assign(edge_len, edge_len + nedges);
     t->degree = degree;
+    for (int p = 0; p < 2; p++)
+        t->clean[p].assign(nr, 0);
     for (int p = 0; p < 2; p++) {
         if ((int)t->cache[p].size() < nr)
             t->cache[p].resize(nr);
@@ -2098,6 +2153,7 @@ int crux_tree_update_rings(crux_tree *t, int nrings, const int32_t *rings,
     t->ring_node.swap(rn);
     t->ring_next.swap(rx);
     t->node_ring.swap(nring);
+    mark_all(t);
     for (auto &d : deg)
         t->degree[d.first] = d.second;
     for (int i = 0; i < nrings; i++)
@@ -2113,6 +2169,9 @@ int crux_tree_set_lengths(crux_tree *t, int n, const int32_t *edges, const doubl
     for (int i = 0; i < n; i++) {
         if (edges[i] < 0 || edges[i] >= t->nedges)
             return set_error(CRUX_EINVAL, "edge %d out of range", edges[i]);
+        // An equal length leaves every key and CL as it was.
+        if (!(t->edge_len[edges[i]] == lens[i]) && t->has_topology)
+            mark_edit(t, edges[i]);
         t->edge_len[edges[i]] = lens[i];
     }
     return CRUX_OK;
