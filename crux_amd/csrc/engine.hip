@@ -321,7 +321,7 @@ struct crux_tree {
     // subtrees: a clean ring's whole subtree is clean.  The planner skips a
     // clean subtree with a valid CL at its top (checking only that ring's
     // own key); a length edit clears the rings whose subtree holds the edge,
-    // a topology edit clears all.
+    // a topology edit clears the rings whose subtree holds a re-linked node.
     std::vector<uint8_t> clean[2];
     Arena arena[2];
     crux_lik *owner[2] = {nullptr, nullptr};
@@ -436,10 +436,42 @@ static void mark_edit(crux_tree *t, int32_t e)
     }
 }
 
-static void mark_all(crux_tree *t)
+// A topology edit that re-linked rings at the given nodes: clear every ring
+// of those nodes and every ring whose subtree holds one of them (walking
+// outward from each node as mark_edit does from an edge).  Stopping at a ring
+// already unclean stays right across the edit: between that ring and one
+// farther out with no edited node in between, the tree is as it was, so the
+// farther ring was unclean already; with an edited node in between, the walk
+// from that node reaches it.
+static void mark_nodes(crux_tree *t, const std::vector<int32_t> &nodes)
 {
-    for (int p = 0; p < 2; p++)
-        std::fill(t->clean[p].begin(), t->clean[p].end(), (uint8_t)0);
+    thread_local std::vector<int32_t> todo;
+    todo.clear();
+    for (int32_t n : nodes) {
+        const int32_t h = t->node_ring[n];
+        if (h < 0)
+            continue;
+        int32_t r = h;
+        do {
+            t->clean[0][r] = 0;
+            t->clean[1][r] = 0;
+            todo.push_back(r ^ 1);
+            r = t->ring_next[r];
+        } while (r != h);
+    }
+    while (!todo.empty()) {
+        const int32_t in = todo.back();
+        todo.pop_back();
+        if (t->ring_node[in] < 0)
+            continue;
+        for (int32_t r = t->ring_next[in]; r != in; r = t->ring_next[r]) {
+            if (!t->clean[0][r] && !t->clean[1][r])
+                continue;
+            t->clean[0][r] = 0;
+            t->clean[1][r] = 0;
+            todo.push_back(r ^ 1);
+        }
+    }
 }
 
 static int arena_layout(crux_tree *t, int pol, int C)
@@ -2153,7 +2185,10 @@ int crux_tree_update_rings(crux_tree *t, int nrings, const int32_t *rings,
     t->ring_node.swap(rn);
     t->ring_next.swap(rx);
     t->node_ring.swap(nring);
-    mark_all(t);
+    for (int i = 0; i < nrings; i++)
+        for (int p = 0; p < 2; p++)
+            t->clean[p][rings[i]] = 0;
+    mark_nodes(t, touched);
     for (auto &d : deg)
         t->degree[d.first] = d.second;
     for (int i = 0; i < nrings; i++)

@@ -1,11 +1,11 @@
 """Incremental planning against planning from scratch.
 
 The planner skips subtrees in which nothing changed since it last walked
-them (a length edit marks the rings whose subtree holds the edge, a topology
-edit marks every ring), checking only the cache key of the ring at the top
-of a skipped subtree: the reference's keys (parent, sibling count, branch
-length; pkg/Crux/Tree/Lik.pyx:1500-1532) decide what is recomputed, as
-before.  Here one likelihood (and its clone, the other polarity) follows a
+them (a length edit marks the rings whose subtree holds the edge, a
+topology edit those whose subtree holds a re-linked node), checking only
+the cache key of the ring at the top of a skipped subtree: the reference's
+keys (parent, sibling count, branch length;
+pkg/Crux/Tree/Lik.pyx:1500-1532) decide what is recomputed, as before.  Here one likelihood (and its clone, the other polarity) follows a
 random sequence of branch-length edits, no-op edits, NNI moves and re-rootings
 without ever being flushed, while a twin on an identical tree receives the
 same edits and is flushed before every evaluation.  A CLV computed from
