@@ -97,6 +97,15 @@ class _DeviceTree:
         self.ids = None
         self.sink = tree.length_sink()
         self.tsink = tree.topology_sink()
+        # Length uploads of a few edges (an MC3 move) reuse these arrays: a
+        # new ctypes array type per call costs more than the call itself.
+        self._nbuf = 16
+        self._idbuf = (nat.ctypes.c_int32 * self._nbuf)()
+        self._lenbuf = (nat.ctypes.c_double * self._nbuf)()
+        self._set_lengths = nat.lib().crux_tree_set_lengths
+        self._tbuf_n = 64
+        self._tbuf = [(nat.ctypes.c_int32 * self._tbuf_n)() for _ in range(5)]
+        self._update_rings = nat.lib().crux_tree_update_rings
 
     def sync(self):
         t = self.tree
@@ -107,19 +116,26 @@ class _DeviceTree:
         if self.sn != t.tsn:
             rings, nodes = self.tsink
             if self.ids == (t.max_node_id, t.max_edge_id):
-                # A few rings and nodes (an MC3 move): plain ctypes arrays
-                # cost far less than numpy ones at that size.
-                rl = sorted(rings, key=lambda r: r.id)
-                nl = sorted(nodes, key=lambda n: n.id)
-                nr, nn = len(rl), len(nl)
-                ri = nat.ctypes.c_int32 * nr
-                ni = nat.ctypes.c_int32 * nn
-                nat.check(nat.lib().crux_tree_update_rings(
-                    self.handle, nr, ri(*[r.id for r in rl]),
-                    ri(*[r.node.id if r.node is not None else -1 for r in rl]),
-                    ri(*[r.next.id if r.node is not None else -1 for r in rl]),
-                    nn, ni(*[n.id for n in nl]),
-                    ni(*[n.ring.id if n.ring is not None else -1 for n in nl])))
+                # A few rings and nodes (an MC3 move), in any order, through
+                # arrays kept for the purpose (see _nbuf).
+                nr, nn = len(rings), len(nodes)
+                if nr <= self._tbuf_n and nn <= self._tbuf_n:
+                    rid, rnode, rnext, nid, nring = self._tbuf
+                else:
+                    rid, rnode, rnext = [(nat.ctypes.c_int32 * nr)() for _ in range(3)]
+                    nid, nring = [(nat.ctypes.c_int32 * nn)() for _ in range(2)]
+                for i, r in enumerate(rings):
+                    rid[i] = r.id
+                    if r.node is not None:
+                        rnode[i] = r.node.id
+                        rnext[i] = r.next.id
+                    else:
+                        rnode[i] = -1
+                        rnext[i] = -1
+                for i, n in enumerate(nodes):
+                    nid[i] = n.id
+                    nring[i] = n.ring.id if n.ring is not None else -1
+                nat.check(self._update_rings(self.handle, nr, rid, rnode, rnext, nn, nid, nring))
             else:
                 rn, rx, nr, nt, el = t.topology_arrays(self.alignment.taxaMap)
                 nat.check(nat.lib().crux_tree_set_topology(
@@ -131,14 +147,21 @@ class _DeviceTree:
             nodes.clear()
             self.sn = t.tsn
         if self.sink:
-            # Usually one or two edges (an MC3 move): plain ctypes arrays
-            # cost less than numpy ones at that size.
-            edges = list(self.sink)
+            # Usually one or two edges (an MC3 move).
+            n = len(self.sink)
+            if n <= self._nbuf:
+                ids, lens = self._idbuf, self._lenbuf
+                for i, e in enumerate(self.sink):
+                    ids[i] = e.id
+                    lens[i] = e._length
+            else:
+                edges = list(self.sink)
+                ids = (nat.ctypes.c_int32 * n)(*[e.id for e in edges])
+                lens = (nat.ctypes.c_double * n)(*[e.length for e in edges])
             self.sink.clear()
-            n = len(edges)
-            ids = (nat.ctypes.c_int32 * n)(*[e.id for e in edges])
-            lens = (nat.ctypes.c_double * n)(*[e.length for e in edges])
-            nat.check(nat.lib().crux_tree_set_lengths(self.handle, n, ids, lens))
+            rc = self._set_lengths(self.handle, n, ids, lens)
+            if rc:
+                nat.check(rc)
 
     def close(self):
         if self.handle:

@@ -255,6 +255,12 @@ def random_lik(tree, alignment, params, rng, context=None):
     return lik
 
 
+def _index(rng, n):
+    """A uniform index in [0, n) from one uniform draw (numpy's scalar
+    Generator.integers costs several times as much per call)."""
+    return min(int(rng.random() * n), n - 1)
+
+
 def _accept_prob(x):
     """exp(x) for an acceptance test, NaN (a failed evaluation) as 0."""
     if x != x:
@@ -496,7 +502,7 @@ class Chain:
     def _p_brlen(self):
         """Chain.pyx:438-475."""
         edges = self._edges()
-        edge = edges[int(self.rng.integers(len(edges)))]
+        edge = edges[_index(self.rng, len(edges))]
         lnM = self._draw_lnm(self.params.brlenLambda)
         v0 = edge.length
         v1 = v0 * math.exp(lnM)
@@ -520,12 +526,12 @@ class Chain:
             [e for e in self._edges() if e.ring.node.degree >= 3 and e.ring.other.node.degree >= 3]
         if not inner:
             return None
-        e = inner[int(self.rng.integers(len(inner)))]
+        e = inner[_index(self.rng, len(inner))]
         u, v = e.ring.node, e.ring.other.node
         ra = [r for r in u.ring if r is not e.ring]
         rb = [r for r in v.ring if r is not e.ring.other]
-        a = ra[int(self.rng.integers(len(ra)))]
-        b = rb[int(self.rng.integers(len(rb)))]
+        a = ra[_index(self.rng, len(ra))]
+        b = rb[_index(self.rng, len(rb))]
         _swap_subtrees(a, b, u, v)
         # a now hangs on v and b on u: swap back on rejection.
         return _Pending("nni", self.lik, u, 0.0, 0.0, None,
@@ -1123,8 +1129,8 @@ class Mc3Batch:
             if len(run) < 2:
                 continue
             rng = self.swap_rngs[r]
-            a = int(rng.integers(len(run)))
-            b = int(rng.integers(len(run) - 1))
+            a = _index(rng, len(run))
+            b = _index(rng, len(run) - 1)
             if b >= a:
                 b += 1
             u = rng.random()
