@@ -22,6 +22,7 @@ CRUX_TUNE_PRECISION = 2
 CRUX_TUNE_CHUNK = 3
 CRUX_TUNE_WAVES = 4
 CRUX_TUNE_SHORT_PLANS = 5
+CRUX_TUNE_SPLIT = 6
 CRUX_PREC_F32, CRUX_PREC_BF16, CRUX_PREC_F16, CRUX_PREC_F16X3 = range(4)
 
 # Mixture components (over all models, +I counted) a likelihood may have:

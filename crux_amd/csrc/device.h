@@ -247,6 +247,9 @@ struct Entry {
     const uint8_t *const *tip_rows;  // padded tip row of each tip child, plan order
     int32_t ntips;
     int32_t exp_off;       // bytes from a CLV slot to its exponents
+    // 20/22 states: the tip chunk staged when the entry starts (an entry
+    // that starts inside a plan starts in that plan's chunk).
+    int32_t tip0;
     // 4-state: P images of the plan, p4_floats(CM) per update with CM the
     // component class (columns of child slots without P are left unused),
     // staged into LDS a chunk of updates at a time.

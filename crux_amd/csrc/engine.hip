@@ -78,6 +78,7 @@ struct crux_ctx {
     int waves20 = 0;               // 20/22-state waves per SIMD (0 = default)
     int prec = kPrecF32;           // input precision of 20/22-state products
     bool short_plans = true;       // 4-state short plans on the short-plan kernel
+    int split20 = 3;               // 20/22-state split: blocks per SIMD aimed at (0: never)
     // Per-class settings of the three shape keys (-1: the context's value),
     // indexed by class_index(K, CM).
     int ctune[kNumClasses][3];
@@ -736,7 +737,7 @@ static int op_shape(const Op &op, const PlanOp &pop)
             return 0;
         src[j] = kind == kChildTip ? kSrcTip
                  : ((pop.fwd >> j) & 1) ? kSrcFwd
-                 : ((op.from_stack >> j) & 1) ? kSrcStk
+                 : ((pop.from_stack >> j) & 1) ? kSrcStk
                                               : kSrcHbm;
     }
     if (src[0] == kSrcFwd && src[1] == kSrcFwd)
@@ -965,6 +966,100 @@ static std::vector<int32_t> assign_tips(std::vector<PlanOp> &ops, bool per_launc
     return taxa;
 }
 
+// A 20/22-state plan run over two launches.  A launch with few blocks per
+// SIMD leaves most SIMDs running one wave that waits on its own loads, and
+// the SIMDs that got a second block set the launch's time; walking each site
+// tile's subtrees as blocks of their own gives the first launch several
+// shorter blocks per tile, started as SIMDs free up, and the second launch
+// the updates above them.
+struct PlanSplit {
+    std::vector<std::pair<int32_t, int32_t>> units;   // first and last update of each subtree
+    std::vector<int32_t> top;                         // the updates above them, in plan order
+    std::vector<uint8_t> from_unit;                   // per update: bit j, child j is a subtree's result
+    std::vector<int32_t> chunk;                       // per update: tip chunk staged when it runs
+};
+
+// Splits a plan into its largest subtrees of at most T updates (each the
+// range of the plan that ends at its root: a plan lists a subtree's updates
+// together, which is checked) and the updates above them, which read the
+// subtrees' results from their slots.  Returns false, leaving sp empty, where
+// that does not pay or does not apply: fewer than two subtrees, more than a
+// third of the updates above them, or a result read by anything but one
+// later update's internal child.
+static bool split_plan(const std::vector<PlanOp> &ops, int T, PlanSplit &sp)
+{
+    sp = PlanSplit{};
+    const int n = (int)ops.size();
+    std::vector<int32_t> size(n, 1), cons(n, -1), cons_kind(n, -1);
+    std::vector<std::array<int32_t, kOpMaxChild>> prod(n);
+    std::unordered_map<int32_t, int32_t> last;   // slot -> the update that last wrote it
+    for (int i = 0; i < n; i++) {
+        prod[i].fill(-1);
+        for (size_t j = 0; j < ops[i].ch.size(); j++) {
+            const PlanChild &c = ops[i].ch[j];
+            if (c.kind == kChildTip)
+                continue;
+            auto it = last.find(c.slot);
+            if (it == last.end())
+                continue;   // cached by an earlier plan
+            const int p = it->second;
+            if (cons[p] >= 0)
+                return false;
+            cons[p] = i;
+            cons_kind[p] = c.kind;
+            prod[i][j] = p;
+            size[i] += size[p];
+            last.erase(it);
+        }
+        if (ops[i].parent_slot >= 0)
+            last[ops[i].parent_slot] = i;
+    }
+    std::vector<int32_t> unit_of(n, -1);
+    std::vector<int32_t> stack;
+    for (int i = 0; i < n; i++) {
+        if (size[i] > T || cons[i] < 0 || size[cons[i]] <= T || cons_kind[i] != kChildInternal ||
+            ops[i].parent_slot < 0)
+            continue;
+        // Its subtree: exactly the size[i] updates ending at i.
+        int lo = i, cnt = 0;
+        stack.assign(1, i);
+        while (!stack.empty()) {
+            const int k = stack.back();
+            stack.pop_back();
+            cnt++;
+            lo = std::min(lo, k);
+            for (int j = 0; j < kOpMaxChild; j++)
+                if (prod[k][j] >= 0)
+                    stack.push_back(prod[k][j]);
+        }
+        if (cnt != size[i] || lo != i - size[i] + 1)
+            return false;
+        for (int k = lo; k <= i; k++)
+            unit_of[k] = (int32_t)sp.units.size();
+        sp.units.push_back({lo, i});
+    }
+    for (int i = 0; i < n; i++)
+        if (unit_of[i] < 0)
+            sp.top.push_back(i);
+    if (sp.units.size() < 2 || 3 * sp.top.size() > (size_t)n) {
+        sp = PlanSplit{};
+        return false;
+    }
+    sp.from_unit.assign(n, 0);
+    for (int i : sp.top)
+        for (int j = 0; j < kOpMaxChild; j++)
+            if (prod[i][j] >= 0 && unit_of[prod[i][j]] >= 0)
+                sp.from_unit[i] |= (uint8_t)(1 << j);
+    sp.chunk.assign(n, 0);
+    int32_t base = 0;
+    for (int i = 0; i < n; i++) {
+        if (ops[i].tip_load >= 0)
+            base = ops[i].tip_load;
+        sp.chunk[i] = base;
+    }
+    return true;
+}
+
 using Staging = crux_ctx::Staging;
 
 // A gfx950 buffer resource descriptor (V#) for raw access to nbytes at p:
@@ -1098,6 +1193,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         int idx;    // output index
         int CM;
         std::vector<int32_t> tips;   // 4-state: taxon row of each tip child
+        PlanSplit split;             // 20/22 states: the plan over two launches
     };
     std::vector<Item> items;
     for (size_t i = 0; i < liks.size(); i++) {
@@ -1158,9 +1254,27 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             cls_shape[ci].depth = kPartDepth;
         }
     }
+    // A 20/22-state class launched with fewer blocks than split20 per SIMD
+    // splits its long plans into subtrees of about the number of updates that
+    // makes that many.
+    int cls_split[kNumClasses] = {};
+    if (!plain && ctx->split20 > 0) {
+        double wu[kNumClasses] = {};
+        for (Item &it : items) {
+            const crux_tree *t = it.L->tree;
+            if (t->K != 4)
+                wu[class_index(t->K, it.CM)] +=
+                    (double)((t->S + tile_sites(t->K, it.CM) - 1) / tile_sites(t->K, it.CM)) *
+                    (double)it.L->ops.size();
+        }
+        const double want = (double)ctx->split20 * 4 * ctx->ncu;
+        for (int ci = 0; ci < kNumClasses; ci++)
+            if (wu[ci] > 0 && (double)cls_tiles[ci] < want)
+                cls_split[ci] = std::max(16, (int)std::ceil(wu[ci] / want));
+    }
     // Sizes.
     size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0, ntips_total = 0;
-    size_t ctl_words = 0;
+    size_t ctl_words = 0, nsplit_ent = 0, nsplit_blk = 0;
     size_t shmem = 0;
     for (Item &it : items) {
         crux_lik *L = it.L;
@@ -1180,6 +1294,13 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                     return set_error(CRUX_EINVAL, "plan has a root update before its end");
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
+        const int T = cls_split[class_index(K, it.CM)];
+        it.split = PlanSplit{};
+        if (T > 0 && L->ops.size() >= 2 * (size_t)T && split_plan(L->ops, T, it.split)) {
+            nsplit_ent += it.split.units.size();
+            nsplit_blk += it.split.units.size() * ((t->S + tile_sites(K, it.CM) - 1) / tile_sites(K, it.CM));
+            ctl_words += it.split.top.size() * kCtlWords20;
+        }
         nops += L->ops.size();
         ctl_words += L->ops.size() * (K != 4 ? kCtlWords20
                                       : cls_shape[class_index(K, it.CM)].part ? kCtlWordsPart
@@ -1204,7 +1325,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             }
         tiles += (t->S + tile_sites(K, it.CM) - 1) / tile_sites(K, it.CM);
     }
-    const size_t nent = items.size() + nplain;
+    const size_t nent = items.size() + nplain + nsplit_ent;
     const size_t off_ent = 0;
     const size_t off_ops = align_up(off_ent + nent * sizeof(Entry), 256);
     const size_t off_jobs = align_up(off_ops + nops * sizeof(Op), 256);
@@ -1216,6 +1337,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         size_t nt = (it.L->tree->S + ts - 1) / ts;
         blk_count += plain ? nt * (it.L->ops.size() + 1) : nt;
     }
+    blk_count += nsplit_blk;
     (void)nblk_total;
     const size_t off_tips = align_up(off_blk + blk_count * sizeof(int32_t), 256);
     const size_t off_ctl = align_up(off_tips + ntips_total * sizeof(void *), 256);
@@ -1301,6 +1423,22 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     };
     std::vector<Launch> launches;
     std::vector<size_t> item_poff(items.size()), item_ntl(items.size());
+    // Blocks of one class for the launch being formed (a new launch where
+    // the class changes).
+    auto add_blocks = [&](int K, int CM, int32_t ent, size_t n) {
+        for (size_t b = 0; b < n; b++)
+            hblk[iblk++] = ent;
+        if (launches.empty() || launches.back().K != K || launches.back().CM != CM)
+            launches.push_back({K, CM, iblk - n, 0});
+        launches.back().nblk += n;
+    };
+    // Entries of split plans' upper updates, launched after all others.
+    struct Deferred {
+        Entry e;
+        int K, CM;
+        size_t ntl;
+    };
+    std::vector<Deferred> deferred;
 
     for (size_t ii = 0; ii < items.size(); ii++) {
         crux_lik *L = items[ii].L;
@@ -1317,6 +1455,70 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         const size_t pdense = ipm;
         if (K == 4)
             ipm += align_up(L->ops.size() * p4_floats(CM), 64);
+        auto write_ctl = [&](int32_t *cw, const Op &op, const PlanOp &pop, int32_t tip_load) {
+            memset(cw, 0, nw * sizeof(int32_t));
+            cw[0] = op.nchild | (op.kinds << 8) | (pop.from_stack << 16) | (pop.fwd << 20) |
+                    ((pop.parent_slot >= 0 ? 1 : 0) << 24) | (op_shape(op, pop) << 25);
+            cw[1] = op.pos;
+            cw[2] = tip_load;
+            // has_parent, and in bits 8+ the children that are this
+            // plan's own results read back from HBM (not loadable ahead).
+            cw[3] = (pop.parent_slot >= 0 ? 1 : 0) | (pop.own << kCtlOwnShift);
+            const uint64_t pp = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.clv(pop.parent_slot) : nullptr);
+            const uint64_t pe = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.exps(pop.parent_slot) : nullptr);
+            cw[4] = (int32_t)(uint32_t)pp;
+            cw[5] = (int32_t)(uint32_t)(pp >> 32);
+            cw[6] = (int32_t)(uint32_t)pe;
+            cw[7] = (int32_t)(uint32_t)(pe >> 32);
+            if (part) {
+                // Every child's slot or tip row, and which of them the
+                // short-plan kernel loads ahead: tip rows, and CLVs
+                // cached by an earlier plan (not forwarded, not on the
+                // stack, not this plan's own results).  A child read
+                // where it is used takes its slot from these words too.
+                const int skip = pop.from_stack | pop.fwd | pop.own;
+                for (int j = 0; j < op.nchild; j++) {
+                    const int kind = (op.kinds >> (2 * j)) & 3;
+                    uint64_t pc = 0;
+                    if (kind == kChildTip) {
+                        pc = (uint64_t)(uintptr_t)op.ch[j].codes;
+                        cw[3] |= 1 << (kCtlTipShift + j);
+                    } else {
+                        pc = (uint64_t)(uintptr_t)op.ch[j].clv;
+                        if (kind == kChildInternal && !((skip >> j) & 1))
+                            cw[3] |= 1 << (kCtlAheadShift + j);
+                    }
+                    cw[kCtlAhead + 2 * j] = (int32_t)(uint32_t)pc;
+                    cw[kCtlAhead + 2 * j + 1] = (int32_t)(uint32_t)(pc >> 32);
+                }
+            }
+            if (K == 4)
+                return;
+            // 20/22 states: buffer descriptors the kernel loads into
+            // scalar registers as they are (see Ctl20 in device.h): the
+            // parent's slot, each child's P image or tip table and each
+            // child's slot; the zero block where there is none (an empty
+            // range for slots, so every load of one reads zero).
+            const uint32_t slot_range = (uint32_t)(A.clv_bytes + (size_t)t->S * sizeof(int32_t));
+            put_vsharp(cw + kCtl20Parent, pop.parent_slot >= 0 ? (const void *)A.clv(pop.parent_slot)
+                                                               : ctx->d_zero,
+                       pop.parent_slot >= 0 ? slot_range : 0u);
+            for (int j = 0; j < kOpMaxChild; j++) {
+                const OpChild &oc = op.ch[j];
+                const bool have = j < op.nchild;
+                // The data's own size, so that an offset past it reads
+                // nothing.
+                const int kind = (op.kinds >> (2 * j)) & 3;
+                const uint32_t dat_range =
+                    !(have && oc.mat) ? (uint32_t)kZeroBytes
+                    : kind == kChildTip ? (uint32_t)((size_t)t->ncodes * CM * KP * sizeof(float))
+                                        : (uint32_t)(img_words(KP, CM, ctx->prec) * sizeof(float));
+                put_vsharp(cw + kCtl20Data + 4 * j, have && oc.mat ? (const void *)oc.mat : ctx->d_zero,
+                           dat_range);
+                put_vsharp(cw + kCtl20Slot + 4 * j, have && oc.clv ? (const void *)oc.clv : ctx->d_zero,
+                           have && oc.clv ? slot_range : 0u);
+            }
+        };
         for (PlanOp &pop : L->ops) {
             const size_t olocal = io - op0;
             Op &op = hops[io++];
@@ -1330,70 +1532,6 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             op.tip_load = pop.tip_load;
             int32_t *cw = (int32_t *)(H + off_ctl) + ictl;
             ictl += nw;
-            auto write_ctl = [&]() {
-                memset(cw, 0, nw * sizeof(int32_t));
-                cw[0] = op.nchild | (op.kinds << 8) | (op.from_stack << 16) | (pop.fwd << 20) |
-                        ((pop.parent_slot >= 0 ? 1 : 0) << 24) | (op_shape(op, pop) << 25);
-                cw[1] = op.pos;
-                cw[2] = op.tip_load;
-                // has_parent, and in bits 8+ the children that are this
-                // plan's own results read back from HBM (not loadable ahead).
-                cw[3] = (pop.parent_slot >= 0 ? 1 : 0) | (pop.own << kCtlOwnShift);
-                const uint64_t pp = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.clv(pop.parent_slot) : nullptr);
-                const uint64_t pe = (uint64_t)(uintptr_t)(pop.parent_slot >= 0 ? A.exps(pop.parent_slot) : nullptr);
-                cw[4] = (int32_t)(uint32_t)pp;
-                cw[5] = (int32_t)(uint32_t)(pp >> 32);
-                cw[6] = (int32_t)(uint32_t)pe;
-                cw[7] = (int32_t)(uint32_t)(pe >> 32);
-                if (part) {
-                    // Every child's slot or tip row, and which of them the
-                    // short-plan kernel loads ahead: tip rows, and CLVs
-                    // cached by an earlier plan (not forwarded, not on the
-                    // stack, not this plan's own results).  A child read
-                    // where it is used takes its slot from these words too.
-                    const int skip = op.from_stack | pop.fwd | pop.own;
-                    for (int j = 0; j < op.nchild; j++) {
-                        const int kind = (op.kinds >> (2 * j)) & 3;
-                        uint64_t pc = 0;
-                        if (kind == kChildTip) {
-                            pc = (uint64_t)(uintptr_t)op.ch[j].codes;
-                            cw[3] |= 1 << (kCtlTipShift + j);
-                        } else {
-                            pc = (uint64_t)(uintptr_t)op.ch[j].clv;
-                            if (kind == kChildInternal && !((skip >> j) & 1))
-                                cw[3] |= 1 << (kCtlAheadShift + j);
-                        }
-                        cw[kCtlAhead + 2 * j] = (int32_t)(uint32_t)pc;
-                        cw[kCtlAhead + 2 * j + 1] = (int32_t)(uint32_t)(pc >> 32);
-                    }
-                }
-                if (K == 4)
-                    return;
-                // 20/22 states: buffer descriptors the kernel loads into
-                // scalar registers as they are (see Ctl20 in device.h): the
-                // parent's slot, each child's P image or tip table and each
-                // child's slot; the zero block where there is none (an empty
-                // range for slots, so every load of one reads zero).
-                const uint32_t slot_range = (uint32_t)(A.clv_bytes + (size_t)t->S * sizeof(int32_t));
-                put_vsharp(cw + kCtl20Parent, pop.parent_slot >= 0 ? (const void *)A.clv(pop.parent_slot)
-                                                                   : ctx->d_zero,
-                           pop.parent_slot >= 0 ? slot_range : 0u);
-                for (int j = 0; j < kOpMaxChild; j++) {
-                    const OpChild &oc = op.ch[j];
-                    const bool have = j < op.nchild;
-                    // The data's own size, so that an offset past it reads
-                    // nothing.
-                    const int kind = (op.kinds >> (2 * j)) & 3;
-                    const uint32_t dat_range =
-                        !(have && oc.mat) ? (uint32_t)kZeroBytes
-                        : kind == kChildTip ? (uint32_t)((size_t)t->ncodes * CM * KP * sizeof(float))
-                                            : (uint32_t)(img_words(KP, CM, ctx->prec) * sizeof(float));
-                    put_vsharp(cw + kCtl20Data + 4 * j, have && oc.mat ? (const void *)oc.mat : ctx->d_zero,
-                               dat_range);
-                    put_vsharp(cw + kCtl20Slot + 4 * j, have && oc.clv ? (const void *)oc.clv : ctx->d_zero,
-                               have && oc.clv ? slot_range : 0u);
-                }
-            };
             if (pop.parent_slot >= 0) {
                 op.parent = A.clv(pop.parent_slot);
                 op.parent_exp = A.exps(pop.parent_slot);
@@ -1445,7 +1583,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                     oc.mat = jb.out;
                 }
             }
-            write_ctl();
+            write_ctl(cw, op, pop, op.tip_load);
         }
         Entry base{};
         base.ops = dops + op0;
@@ -1473,15 +1611,48 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             itip += items[ii].tips.size();
         }
         itile += ntl;
-        if (!plain) {
+        const PlanSplit &sp = items[ii].split;
+        if (!plain && !sp.units.empty()) {
+            // The subtrees run on the plan's own records; the updates above
+            // them on records of their own, which read the subtrees' results
+            // from their slots and stage the tip chunks the plan had staged
+            // for them.
+            for (const auto &u : sp.units) {
+                Entry e = base;
+                e.ops = base.ops + u.first;
+                e.ctl = base.ctl + (size_t)u.first * nw;
+                e.nops = u.second - u.first + 1;
+                e.tip0 = sp.chunk[u.first];
+                e.partials = nullptr;
+                e.tile0 = (int32_t)iblk;
+                hent[ient] = e;
+                add_blocks(K, items[ii].CM, (int32_t)ient, ntl);
+                ient++;
+            }
+            Entry top = base;
+            top.ctl = (const int32_t *)(D + off_ctl) + ictl;
+            top.nops = (int32_t)sp.top.size();
+            top.tip0 = sp.chunk[sp.top[0]];
+            for (size_t k = 0; k < sp.top.size(); k++) {
+                const int i = sp.top[k];
+                PlanOp p2 = L->ops[i];
+                const int m = sp.from_unit[i];
+                // A subtree's result the plan handed over in registers or on
+                // the stack is read from its slot (a read the traffic count,
+                // the fused plan's floor, leaves out).
+                p2.from_stack &= ~m;
+                p2.fwd &= ~m;
+                p2.own &= ~m;
+                const int32_t tl = k > 0 && sp.chunk[i] != sp.chunk[sp.top[k - 1]] ? sp.chunk[i] : -1;
+                write_ctl((int32_t *)(H + off_ctl) + ictl, hops[op0 + i], p2, tl);
+                ictl += nw;
+            }
+            deferred.push_back({top, K, items[ii].CM, ntl});
+        } else if (!plain) {
             base.tile0 = (int32_t)iblk;
             hent[ient] = base;
-            for (size_t b = 0; b < ntl; b++)
-                hblk[iblk++] = (int32_t)ient;
+            add_blocks(K, items[ii].CM, (int32_t)ient, ntl);
             ient++;
-            if (launches.empty() || launches.back().K != K || launches.back().CM != items[ii].CM)
-                launches.push_back({K, items[ii].CM, iblk - ntl, 0});
-            launches.back().nblk += ntl;
         } else {
             // One launch per node update, each on a one-op entry.
             for (size_t o = 0; o < L->ops.size(); o++) {
@@ -1503,6 +1674,16 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                 ient++;
             }
         }
+    }
+
+    for (size_t d = 0; d < deferred.size(); d++) {
+        Deferred &df = deferred[d];
+        df.e.tile0 = (int32_t)iblk;
+        hent[ient] = df.e;
+        if (d == 0)
+            launches.push_back({df.K, df.CM, iblk, 0});
+        add_blocks(df.K, df.CM, (int32_t)ient, df.ntl);
+        ient++;
     }
 
     hipStream_t st = ctx->stream;
@@ -1818,6 +1999,12 @@ int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value)
         ctx->short_plans = value != 0;
         return CRUX_OK;
     }
+    if (key == CRUX_TUNE_SPLIT) {
+        if (value < 0 || value > 16)
+            return set_error(CRUX_EINVAL, "split %d (0 to 16)", value);
+        ctx->split20 = value;
+        return CRUX_OK;
+    }
     if (int rc = check_shape_value(key, value))
         return rc;
     *ctx_shape_field(ctx, key) = value;
@@ -1834,6 +2021,10 @@ int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value)
     }
     if (key == CRUX_TUNE_SHORT_PLANS) {
         *value = ctx->short_plans ? 1 : 0;
+        return CRUX_OK;
+    }
+    if (key == CRUX_TUNE_SPLIT) {
+        *value = ctx->split20;
         return CRUX_OK;
     }
     if (key != CRUX_TUNE_STACK_DEPTH && key != CRUX_TUNE_CHUNK && key != CRUX_TUNE_WAVES)

@@ -2124,7 +2124,7 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     Rec r0 = rec(0);
     Rec r1 = rec(min(1, nops - 1));
     if (ntips > 0)
-        stage_tips(0);
+        stage_tips(E->tip0);
     if (nops > 1) {
         const int sh = shape_of(r0.w0());
         if (sh != 0) {
@@ -2240,7 +2240,9 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
             cmp = bb;
         }
     }
-    if (lane == 0) {
+    // An entry that computes part of a plan for a later launch has no
+    // partial sums.
+    if (lane == 0 && E->partials != nullptr) {
         gas(E->partials)[2 * tile] = sum;
         gas(E->partials)[2 * tile + 1] = cmp;
     }

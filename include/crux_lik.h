@@ -278,6 +278,13 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  *   the whole plan at once and loads tip rows and cached children two
  *   updates ahead; 0 runs them on the general kernel.  Same results, bit
  *   for bit, either way.
+ *   CRUX_TUNE_SPLIT: blocks per SIMD a 20/22-state class launch aims at
+ *   (1 to 16, default 3; 0 never splits).  A class launched with fewer site
+ *   tiles than that runs each long plan (at least twice the subtree size) as
+ *   its largest subtrees of about (updates x tiles) / (value x SIMDs)
+ *   updates, one block per subtree and tile in a first launch, and the
+ *   updates above them in a second launch that reads the subtrees' results
+ *   from their slots.  Same results, bit for bit, either way.
  * Settings apply to batches submitted after the call; do not change them
  * while another thread submits on the context. */
 enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
@@ -286,7 +293,8 @@ enum {
     CRUX_TUNE_PRECISION = 2,
     CRUX_TUNE_CHUNK = 3,
     CRUX_TUNE_WAVES = 4,
-    CRUX_TUNE_SHORT_PLANS = 5
+    CRUX_TUNE_SHORT_PLANS = 5,
+    CRUX_TUNE_SPLIT = 6
 };
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
