@@ -1617,7 +1617,16 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             // them on records of their own, which read the subtrees' results
             // from their slots and stage the tip chunks the plan had staged
             // for them.
-            for (const auto &u : sp.units) {
+            // Largest subtrees first: the launch's last blocks to start are
+            // its shortest.
+            std::vector<size_t> order(sp.units.size());
+            for (size_t k = 0; k < order.size(); k++)
+                order[k] = k;
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+                return sp.units[a].second - sp.units[a].first > sp.units[b].second - sp.units[b].first;
+            });
+            for (size_t k : order) {
+                const auto &u = sp.units[k];
                 Entry e = base;
                 e.ops = base.ops + u.first;
                 e.ctl = base.ctl + (size_t)u.first * nw;
@@ -1633,6 +1642,14 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             top.ctl = (const int32_t *)(D + off_ctl) + ictl;
             top.nops = (int32_t)sp.top.size();
             top.tip0 = sp.chunk[sp.top[0]];
+            // Updates above the subtrees with no tip child stage no tips.
+            int top_tips = 0;
+            for (int i : sp.top)
+                top_tips += L->ops[i].tip_count;
+            if (top_tips == 0) {
+                top.ntips = 0;
+                top.tip_rows = nullptr;
+            }
             for (size_t k = 0; k < sp.top.size(); k++) {
                 const int i = sp.top[k];
                 PlanOp p2 = L->ops[i];
