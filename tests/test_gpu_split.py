@@ -153,14 +153,17 @@ def test_split_polytomies():
 
 
 def test_split_batch_of_trees():
-    """Several trees of one class in one batch: every tree's subtrees in the
-    first launch, every upper plan in the second."""
+    """Several trees in one batch: amino-acid trees of two component classes
+    (their subtrees in each class's first launch, their upper updates in a
+    second launch per class after every first one), a tree too small to
+    split (whole, in the first launch) and a DNA tree (one 4-state launch)."""
     say("split batch")
-    probs = [synth.make_problem(n, s, 20 + i, kind="aa", ncat=4)
-             for i, (n, s) in enumerate([(64, 600), (100, 900), (40, 300), (130, 1200)])]
+    spec = [(64, 600, "aa", 4), (100, 900, "aa", 4), (40, 300, "aa", 4), (130, 1200, "aa", 4),
+            (12, 200, "aa", 4), (80, 900, "aa", 1), (50, 2000, "dna", 4)]
+    probs = [synth.make_problem(n, s, 20 + i, kind=k, ncat=c) for i, (n, s, k, c) in enumerate(spec)]
     liks = []
-    for p in probs:
-        L = Lik(p.tree, p.alignment, ncat=4)
+    for p, (_, _, _, c) in zip(probs, spec):
+        L = Lik(p.tree, p.alignment, ncat=c)
         synth.configure(L, p.model)
         liks.append(L)
     ctx = nat.get_context(0)
@@ -176,11 +179,12 @@ def test_split_batch_of_trees():
                           [np.array(L.siteLnLs()) for L in liks])
     finally:
         ctx.set_tuning(nat.CRUX_TUNE_SPLIT, 3)
-    assert out[0][1] == 1 and out[3][1] == 2, (out[0][1], out[3][1])
+    # One launch per class unsplit; one more per amino-acid class split.
+    assert out[0][1] == 3 and out[3][1] == 5, (out[0][1], out[3][1])
     assert np.array_equal(out[0][0].view(np.uint64), out[3][0].view(np.uint64))
     for a, b in zip(out[0][2], out[3][2]):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
-    for p, v in zip(probs, out[3][0]):
-        ora = OracleLik(p.fresh_tree(), p.alignment, ncat=4, nthreads=4)
+    for p, (_, _, _, c), v in zip(probs, spec, out[3][0]):
+        ora = OracleLik(p.fresh_tree(), p.alignment, ncat=c, nthreads=4)
         synth.configure(ora, p.model)
         assert abs(v - ora.lnL()) <= REL * abs(v)
