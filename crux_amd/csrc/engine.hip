@@ -1254,9 +1254,12 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             cls_shape[ci].depth = kPartDepth;
         }
     }
-    // A 20/22-state class launched with fewer blocks than split20 per SIMD
-    // splits its long plans into subtrees of about the number of updates that
-    // makes that many.
+    // A 20/22-state class launched with fewer site tiles than split20 per
+    // SIMD splits its long plans into subtrees of about the number of updates
+    // that makes that many blocks (at most half a plan); so does one with
+    // fewer than twice that many tiles and at least 200 x split20 updates of
+    // work per SIMD (enough for the balance the subtree blocks bring to
+    // outweigh the second launch).
     int cls_split[kNumClasses] = {};
     if (!plain && ctx->split20 > 0) {
         double wu[kNumClasses] = {};
@@ -1267,9 +1270,10 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                     (double)((t->S + tile_sites(t->K, it.CM) - 1) / tile_sites(t->K, it.CM)) *
                     (double)it.L->ops.size();
         }
-        const double want = (double)ctx->split20 * 4 * ctx->ncu;
+        const double simds = 4.0 * ctx->ncu, want = (double)ctx->split20 * simds;
         for (int ci = 0; ci < kNumClasses; ci++)
-            if (wu[ci] > 0 && (double)cls_tiles[ci] < want)
+            if (wu[ci] > 0 && ((double)cls_tiles[ci] < want ||
+                               ((double)cls_tiles[ci] < 2 * want && wu[ci] >= 200 * want)))
                 cls_split[ci] = std::max(16, (int)std::ceil(wu[ci] / want));
     }
     // Sizes.
@@ -1294,9 +1298,10 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                     return set_error(CRUX_EINVAL, "plan has a root update before its end");
         it.tips = assign_tips(L->ops, plain);
         ntips_total += it.tips.size();
-        const int T = cls_split[class_index(K, it.CM)];
+        // Subtrees of at most half the plan, so that there are at least two.
+        const int T = std::min(cls_split[class_index(K, it.CM)], (int)(L->ops.size() / 2));
         it.split = PlanSplit{};
-        if (T > 0 && L->ops.size() >= 2 * (size_t)T && split_plan(L->ops, T, it.split)) {
+        if (T >= 16 && split_plan(L->ops, T, it.split)) {
             nsplit_ent += it.split.units.size();
             nsplit_blk += it.split.units.size() * ((t->S + tile_sites(K, it.CM) - 1) / tile_sites(K, it.CM));
             ctl_words += it.split.top.size() * kCtlWords20;

@@ -280,11 +280,13 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  *   for bit, either way.
  *   CRUX_TUNE_SPLIT: blocks per SIMD a 20/22-state class launch aims at
  *   (1 to 16, default 3; 0 never splits).  A class launched with fewer site
- *   tiles than that runs each long plan (at least twice the subtree size) as
- *   its largest subtrees of about (updates x tiles) / (value x SIMDs)
- *   updates, one block per subtree and tile in a first launch, and the
- *   updates above them in a second launch that reads the subtrees' results
- *   from their slots.  Same results, bit for bit, either way.
+ *   tiles than that per SIMD (or fewer than twice that many, with at least
+ *   200 x value updates of work per SIMD) runs each plan of 32 updates or
+ *   more as its largest subtrees of at most (updates x tiles) / (value x
+ *   SIMDs) updates and half the plan, one block per subtree and tile in a
+ *   first launch, and the updates above them in a second launch that reads
+ *   the subtrees' results from their slots.  Same results, bit for bit,
+ *   either way.
  * Settings apply to batches submitted after the call; do not change them
  * while another thread submits on the context. */
 enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
