@@ -1263,18 +1263,23 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     int cls_split[kNumClasses] = {};
     if (!plain && ctx->split20 > 0) {
         double wu[kNumClasses] = {};
+        int cls_k[kNumClasses] = {};
         for (Item &it : items) {
             const crux_tree *t = it.L->tree;
-            if (t->K != 4)
-                wu[class_index(t->K, it.CM)] +=
-                    (double)((t->S + tile_sites(t->K, it.CM) - 1) / tile_sites(t->K, it.CM)) *
-                    (double)it.L->ops.size();
+            const int ci = class_index(t->K, it.CM);
+            cls_k[ci] = t->K;
+            if (t->K != 4 || !cls_shape[ci].part)
+                wu[ci] += (double)((t->S + tile_sites(t->K, it.CM) - 1) / tile_sites(t->K, it.CM)) *
+                          (double)it.L->ops.size();
         }
-        const double simds = 4.0 * ctx->ncu, want = (double)ctx->split20 * simds;
-        for (int ci = 0; ci < kNumClasses; ci++)
+        // 20/22 states: one-wave blocks against SIMDs; 4 states: blocks of
+        // one 64-site tile against CUs.
+        for (int ci = 0; ci < kNumClasses; ci++) {
+            const double want = (double)ctx->split20 * (cls_k[ci] != 4 ? 4.0 * ctx->ncu : (double)ctx->ncu);
             if (wu[ci] > 0 && ((double)cls_tiles[ci] < want ||
                                ((double)cls_tiles[ci] < 2 * want && wu[ci] >= 200 * want)))
                 cls_split[ci] = std::max(16, (int)std::ceil(wu[ci] / want));
+        }
     }
     // Sizes.
     size_t nops = 0, njobs = 0, pm_floats = 0, tiles = 0, nplain = 0, ntips_total = 0;
@@ -1304,7 +1309,16 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         if (T >= 16 && split_plan(L->ops, T, it.split)) {
             nsplit_ent += it.split.units.size();
             nsplit_blk += it.split.units.size() * ((t->S + tile_sites(K, it.CM) - 1) / tile_sites(K, it.CM));
-            ctl_words += it.split.top.size() * kCtlWords20;
+            ctl_words += it.split.top.size() * (K != 4 ? kCtlWords20 : kCtlWords);
+            if (K == 4) {
+                // The upper updates' own Op copies and P images (their
+                // children's P matrices computed a second time).
+                nops += it.split.top.size();
+                pm_floats += align_up(it.split.top.size() * p4_floats(it.CM), 64);
+                for (int i : it.split.top)
+                    for (const PlanChild &c : L->ops[i].ch)
+                        njobs += c.kind != kChildIdent;
+            }
         }
         nops += L->ops.size();
         ctl_words += L->ops.size() * (K != 4 ? kCtlWords20
@@ -1444,6 +1458,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         size_t ntl;
     };
     std::vector<Deferred> deferred;
+    std::vector<size_t> job_first;
 
     for (size_t ii = 0; ii < items.size(); ii++) {
         crux_lik *L = items[ii].L;
@@ -1460,6 +1475,8 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         const size_t pdense = ipm;
         if (K == 4)
             ipm += align_up(L->ops.size() * p4_floats(CM), 64);
+        std::vector<size_t> &job0 = job_first;   // first P job of each update
+        job0.assign(L->ops.size() + 1, 0);
         auto write_ctl = [&](int32_t *cw, const Op &op, const PlanOp &pop, int32_t tip_load) {
             memset(cw, 0, nw * sizeof(int32_t));
             cw[0] = op.nchild | (op.kinds << 8) | (pop.from_stack << 16) | (pop.fwd << 20) |
@@ -1526,6 +1543,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         };
         for (PlanOp &pop : L->ops) {
             const size_t olocal = io - op0;
+            job0[olocal] = ij;
             Op &op = hops[io++];
             memset(&op, 0, sizeof op);
             op.nchild = (int32_t)pop.ch.size();
@@ -1590,6 +1608,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             }
             write_ctl(cw, op, pop, op.tip_load);
         }
+        job0[L->ops.size()] = ij;
         Entry base{};
         base.ops = dops + op0;
         base.ctl = (const int32_t *)(D + off_ctl) + ctl0;
@@ -1635,8 +1654,12 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                 Entry e = base;
                 e.ops = base.ops + u.first;
                 e.ctl = base.ctl + (size_t)u.first * nw;
+                if (K == 4)
+                    e.pdense = base.pdense + (size_t)u.first * p4_floats(CM);
                 e.nops = u.second - u.first + 1;
                 e.tip0 = sp.chunk[u.first];
+                if (e.ntips > 0 && e.tip0 >= e.ntips)
+                    return set_error(CRUX_EINVAL, "split plan: tip chunk %d outside %d tips", e.tip0, e.ntips);
                 e.partials = nullptr;
                 e.tile0 = (int32_t)iblk;
                 hent[ient] = e;
@@ -1646,11 +1669,19 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
             Entry top = base;
             top.ctl = (const int32_t *)(D + off_ctl) + ictl;
             top.nops = (int32_t)sp.top.size();
-            top.tip0 = sp.chunk[sp.top[0]];
-            // Updates above the subtrees with no tip child stage no tips.
+            // Tip chunks: the one of the first update with a tip child is
+            // staged when the entry starts, another only where an update
+            // with a tip child needs it; updates above the subtrees with no
+            // tip child at all stage nothing (and have no tip list).
             int top_tips = 0;
+            int32_t staged = -1;
             for (int i : sp.top)
-                top_tips += L->ops[i].tip_count;
+                if (L->ops[i].tip_count > 0) {
+                    top_tips += L->ops[i].tip_count;
+                    if (staged < 0)
+                        staged = sp.chunk[i];
+                }
+            top.tip0 = staged < 0 ? 0 : staged;
             if (top_tips == 0) {
                 top.ntips = 0;
                 top.tip_rows = nullptr;
@@ -1665,9 +1696,35 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                 p2.from_stack &= ~m;
                 p2.fwd &= ~m;
                 p2.own &= ~m;
-                const int32_t tl = k > 0 && sp.chunk[i] != sp.chunk[sp.top[k - 1]] ? sp.chunk[i] : -1;
+                int32_t tl = -1;
+                if (L->ops[i].tip_count > 0 && sp.chunk[i] != staged) {
+                    tl = sp.chunk[i];
+                    staged = tl;
+                }
+                // A staged chunk lies inside the tip list (the kernels read
+                // its rows from there).
+                if (tl >= 0 && (top.ntips <= tl || top.tip_rows == nullptr))
+                    return set_error(CRUX_EINVAL, "split plan: tip chunk %d outside %d tips", tl, top.ntips);
                 write_ctl((int32_t *)(H + off_ctl) + ictl, hops[op0 + i], p2, tl);
                 ictl += nw;
+            }
+            if (K == 4) {
+                // The 4-state kernels read the updates' Ops and P images in
+                // plan order: the upper updates get contiguous copies, their
+                // P images from jobs of their own.
+                const size_t o_top = io, pm_top = ipm;
+                ipm += align_up(sp.top.size() * p4_floats(CM), 64);
+                for (size_t k = 0; k < sp.top.size(); k++) {
+                    const int i = sp.top[k];
+                    hops[io++] = hops[op0 + i];
+                    for (size_t q = job0[i]; q < job0[i + 1]; q++) {
+                        PJob &jb = hjobs[ij++];
+                        jb = hjobs[q];
+                        jb.out = S.d_pmat + pm_top + k * p4_floats(CM);
+                    }
+                }
+                top.ops = dops + o_top;
+                top.pdense = S.d_pmat + pm_top;
             }
             deferred.push_back({top, K, items[ii].CM, ntl});
         } else if (!plain) {

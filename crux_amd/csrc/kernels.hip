@@ -508,7 +508,7 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
     if (nops > PC)
         fetch(1);
     if (ntips > 0)
-        stage_tips(0);
+        stage_tips(E->tip0);
 
     // Control words and P image (one float per lane and register) of the
     // update about to run.
@@ -715,7 +715,7 @@ k_trav4(const Entry *__restrict__ entries, const int32_t *__restrict__ block_ent
             cmp = bb;
         }
     }
-    if (lane == 0) {
+    if (lane == 0 && E->partials != nullptr) {
         gas(E->partials)[2 * tile] = sum;
         gas(E->partials)[2 * tile + 1] = cmp;
     }
@@ -838,6 +838,9 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         constexpr int kRows = 64 / kDw;              // rows per instruction
         constexpr int kInstr = kTipChunk / kRows;
         const int n = min(kTipChunk, ntips - base);
+        // A chunk outside the list has no rows to read.
+        if (n <= 0)
+            return;
         const CRUX_GAS uint8_t *const CRUX_GAS *rows =
             (const CRUX_GAS uint8_t *const CRUX_GAS *)gas(E->tip_rows) + base;
         const int sub = lane / kDw, dw = lane % kDw;
@@ -872,7 +875,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     if (nchunks > 1)
         fetch(1);
     if (ntips > 0)
-        stage_tips(0);
+        stage_tips(E->tip0);
 
     // Control words and P image of the update about to run.
     int4 cwa, cwb;
@@ -1227,7 +1230,7 @@ k_trav4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         red[1] = cmp;
     }
     lds_barrier();
-    if (wv == 0 && lane == 0) {
+    if (wv == 0 && lane == 0 && E->partials != nullptr) {
         nsum_merge(sum, cmp, red[0], red[1]);
         gas(E->partials)[2 * tile] = sum;
         gas(E->partials)[2 * tile + 1] = cmp;
@@ -1605,7 +1608,7 @@ k_part4h(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         red[1] = cmp;
     }
     lds_barrier();
-    if (wv == 0 && lane == 0) {
+    if (wv == 0 && lane == 0 && E->partials != nullptr) {
         nsum_merge(sum, cmp, red[0], red[1]);
         gas(E->partials)[2 * tile] = sum;
         gas(E->partials)[2 * tile + 1] = cmp;

@@ -1,9 +1,10 @@
-"""20/22-state plans split over two launches against one launch.
+"""Plans split over two launches against one launch.
 
-A 20/22-state class launched with few site tiles per SIMD runs each long
-plan as its largest subtrees of about T updates (one block per subtree and
-tile, first launch) and the updates above them (second launch, reading the
-subtrees' results from their slots).  Every CLV is the same product of the
+A class launched with few site tiles (per SIMD for 20/22 states, per CU
+for 4 states) runs each long plan as its largest subtrees of about T
+updates (one block per subtree and tile, first launch) and the updates
+above them (second launch, reading the subtrees' results from their
+slots).  Every CLV is the same product of the
 same inputs either way, so every site lnL must be equal bit for bit; the
 oracle checks the split results within the FP32-storage tolerance.
 
@@ -100,16 +101,20 @@ def _check(prob, ncat, seed, depths=(0,), tree=None, splits=(3, 1, 16)):
     _oracle(prob, ncat, on, tree)
 
 
-@pytest.mark.parametrize("ntaxa,nsites,ncat,seed", [
-    (120, 1500, 4, 1),   # four components, tips in two staged chunks
-    (90, 900, 1, 2),     # one component
-    (70, 800, 2, 3),     # two components
-    (60, 700, 3, 4),     # class of four, one idle component
-    (50, 500, 6, 5),     # class of eight, two idle components
+@pytest.mark.parametrize("ntaxa,nsites,ncat,seed,kind", [
+    (120, 1500, 4, 1, "aa"),    # four components, tips in two staged chunks
+    (90, 900, 1, 2, "aa"),      # one component
+    (70, 800, 2, 3, "aa"),      # two components
+    (60, 700, 3, 4, "aa"),      # class of four, one idle component
+    (50, 500, 6, 5, "aa"),      # class of eight, two idle components
+    (150, 3000, 4, 11, "dna"),  # 4 states: lane-halves kernel, four components
+    (100, 2000, 1, 12, "dna"),  # one-component 4-state kernel
+    (80, 1500, 2, 13, "dna"),   # two components
+    (60, 1200, 6, 14, "dna"),   # class of eight, two idle components
 ])
-def test_split_matches_one_launch(ntaxa, nsites, ncat, seed):
-    say("split", ntaxa, nsites, ncat, seed)
-    prob = synth.make_problem(ntaxa, nsites, seed, kind="aa", ncat=ncat)
+def test_split_matches_one_launch(ntaxa, nsites, ncat, seed, kind):
+    say("split", ntaxa, nsites, ncat, seed, kind)
+    prob = synth.make_problem(ntaxa, nsites, seed, kind=kind, ncat=ncat)
     _check(prob, ncat, seed, depths=(0, 2, 8) if ncat == 4 else (0,))
 
 
@@ -132,11 +137,12 @@ def test_split_protein_22_states():
     _check(prob, 4, 7)
 
 
-def test_split_polytomies():
+@pytest.mark.parametrize("kind", ["aa", "dna"])
+def test_split_polytomies(kind):
     """Nodes of degree 5 and more (identity children) and a re-rooting at the
     largest one."""
-    say("split polytomies")
-    prob = synth.make_problem(110, 900, 8, kind="aa", ncat=4)
+    say("split polytomies", kind)
+    prob = synth.make_problem(110, 900 if kind == "aa" else 2500, 8, kind=kind, ncat=4)
     rng = np.random.default_rng(8)
     trees = [prob.tree, prob.fresh_tree()]
     edges0 = list(trees[0].edges)
@@ -156,7 +162,7 @@ def test_split_batch_of_trees():
     """Several trees in one batch: amino-acid trees of two component classes
     (their subtrees in each class's first launch, their upper updates in a
     second launch per class after every first one), a tree too small to
-    split (whole, in the first launch) and a DNA tree (one 4-state launch)."""
+    split (whole, in the first launch) and a DNA tree (split likewise)."""
     say("split batch")
     spec = [(64, 600, "aa", 4), (100, 900, "aa", 4), (40, 300, "aa", 4), (130, 1200, "aa", 4),
             (12, 200, "aa", 4), (80, 900, "aa", 1), (50, 2000, "dna", 4)]
@@ -179,8 +185,8 @@ def test_split_batch_of_trees():
                           [np.array(L.siteLnLs()) for L in liks])
     finally:
         ctx.set_tuning(nat.CRUX_TUNE_SPLIT, 3)
-    # One launch per class unsplit; one more per amino-acid class split.
-    assert out[0][1] == 3 and out[3][1] == 5, (out[0][1], out[3][1])
+    # One launch per class unsplit; one more per class split.
+    assert out[0][1] == 3 and out[3][1] == 6, (out[0][1], out[3][1])
     assert np.array_equal(out[0][0].view(np.uint64), out[3][0].view(np.uint64))
     for a, b in zip(out[0][2], out[3][2]):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
