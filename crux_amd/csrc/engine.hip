@@ -1273,9 +1273,10 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
                           (double)it.L->ops.size();
         }
         // 20/22 states: one-wave blocks against SIMDs; 4 states: blocks of
-        // one 64-site tile against CUs.
+        // one 64-site tile against half the CUs (a CU holds several of them
+        // at once, and near one block per CU splitting no longer pays).
         for (int ci = 0; ci < kNumClasses; ci++) {
-            const double want = (double)ctx->split20 * (cls_k[ci] != 4 ? 4.0 * ctx->ncu : (double)ctx->ncu);
+            const double want = (double)ctx->split20 * (cls_k[ci] != 4 ? 4.0 * ctx->ncu : 0.5 * ctx->ncu);
             if (wu[ci] > 0 && ((double)cls_tiles[ci] < want ||
                                ((double)cls_tiles[ci] < 2 * want && wu[ci] >= 200 * want)))
                 cls_split[ci] = std::max(16, (int)std::ceil(wu[ci] / want));

@@ -278,15 +278,16 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  *   the whole plan at once and loads tip rows and cached children two
  *   updates ahead; 0 runs them on the general kernel.  Same results, bit
  *   for bit, either way.
- *   CRUX_TUNE_SPLIT: blocks per SIMD a 20/22-state class launch aims at
- *   (1 to 16, default 3; 0 never splits).  A class launched with fewer site
- *   tiles than that per SIMD (or fewer than twice that many, with at least
- *   200 x value updates of work per SIMD) runs each plan of 32 updates or
- *   more as its largest subtrees of at most (updates x tiles) / (value x
- *   SIMDs) updates and half the plan, one block per subtree and tile in a
- *   first launch, and the updates above them in a second launch that reads
- *   the subtrees' results from their slots.  Same results, bit for bit,
- *   either way.
+ *   CRUX_TUNE_SPLIT: blocks a class launch aims at, per SIMD for 20/22
+ *   states and per two CUs for 4 states (1 to 16, default 3; 0 never
+ *   splits).  A class launched with fewer site tiles than that (or fewer
+ *   than twice that many, with at least 200 updates of work per block
+ *   aimed at) runs each plan of 32 updates or more, outside the short-plan
+ *   kernel, as its largest subtrees of at most (updates x tiles) / (blocks
+ *   aimed at) updates and half the plan, one block per subtree and tile in
+ *   a first launch, and the updates above them in a second launch that
+ *   reads the subtrees' results from their slots.  Same results, bit for
+ *   bit, either way.
  * Settings apply to batches submitted after the call; do not change them
  * while another thread submits on the context. */
 enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
