@@ -1274,11 +1274,17 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         }
         // 20/22 states: one-wave blocks against SIMDs; 4 states: blocks of
         // one 64-site tile against half the CUs (a CU holds several of them
-        // at once, and near one block per CU splitting no longer pays).
+        // at once, and near one block per CU splitting no longer pays),
+        // or any number of tiles once the launch is long enough for the
+        // balance to outweigh the second launch (400 updates per block
+        // aimed at: 154k at 256 CUs, where a 128-taxon plan over 947 tiles
+        // has 119k and runs 12% slower split).
         for (int ci = 0; ci < kNumClasses; ci++) {
-            const double want = (double)ctx->split20 * (cls_k[ci] != 4 ? 4.0 * ctx->ncu : 0.5 * ctx->ncu);
-            if (wu[ci] > 0 && ((double)cls_tiles[ci] < want ||
-                               ((double)cls_tiles[ci] < 2 * want && wu[ci] >= 200 * want)))
+            const bool k4 = cls_k[ci] == 4;
+            const double want = (double)ctx->split20 * (k4 ? 0.5 * ctx->ncu : 4.0 * ctx->ncu);
+            const double tl = (double)cls_tiles[ci];
+            if (wu[ci] > 0 && (tl < want || (k4 ? wu[ci] >= 400 * want
+                                                 : tl < 2 * want && wu[ci] >= 200 * want)))
                 cls_split[ci] = std::max(16, (int)std::ceil(wu[ci] / want));
         }
     }
