@@ -992,24 +992,28 @@ static bool split_plan(const std::vector<PlanOp> &ops, int T, PlanSplit &sp)
     const int n = (int)ops.size();
     std::vector<int32_t> size(n, 1), cons(n, -1), cons_kind(n, -1);
     std::vector<std::array<int32_t, kOpMaxChild>> prod(n);
-    std::unordered_map<int32_t, int32_t> last;   // slot -> the update that last wrote it
+    int32_t nslot = 0;
+    for (const PlanOp &op : ops) {
+        nslot = std::max(nslot, op.parent_slot + 1);
+        for (const PlanChild &c : op.ch)
+            if (c.kind != kChildTip)
+                nslot = std::max(nslot, c.slot + 1);
+    }
+    std::vector<int32_t> last(nslot, -1);   // slot -> the update that last wrote it
     for (int i = 0; i < n; i++) {
         prod[i].fill(-1);
         for (size_t j = 0; j < ops[i].ch.size(); j++) {
             const PlanChild &c = ops[i].ch[j];
-            if (c.kind == kChildTip)
-                continue;
-            auto it = last.find(c.slot);
-            if (it == last.end())
-                continue;   // cached by an earlier plan
-            const int p = it->second;
+            if (c.kind == kChildTip || c.slot < 0 || last[c.slot] < 0)
+                continue;   // a tip, or cached by an earlier plan
+            const int p = last[c.slot];
             if (cons[p] >= 0)
                 return false;
             cons[p] = i;
             cons_kind[p] = c.kind;
             prod[i][j] = p;
             size[i] += size[p];
-            last.erase(it);
+            last[c.slot] = -1;
         }
         if (ops[i].parent_slot >= 0)
             last[ops[i].parent_slot] = i;
