@@ -97,7 +97,21 @@ __host__ __device__ constexpr int img_groups(int KP, int CM) { return (KP + img_
 // c * (64/CM) + 4 a + i, P_c[4 I + i][g * cols + a] (0 outside the matrix or
 // for c >= C).
 __host__ __device__ constexpr int img_nv(int KP, int CM) { return (KP / 4) * img_groups(KP, CM); }
-__host__ __device__ constexpr int img_floats(int KP, int CM) { return ((img_nv(KP, CM) + 3) & ~3) * 64; }
+// In memory, registers 4q .. 4q + 3 of every lane form 1 KiB blocks of
+// float4 ([q][64 lanes][4]).  Where the traversal loads the last, partly
+// used block past a CLV's float4s (img_tail > 0: the image's last
+// img_nv % 4 registers), those registers follow as [64 lanes][img_tail]
+// dwords instead, so that the load moves what it uses; otherwise the last
+// block is padded.
+__host__ __device__ constexpr int img_tail(int KP, int CM)
+{
+    return (img_nv(KP, CM) % 4 != 0 && (img_nv(KP, CM) + 3) / 4 - 1 >= KP / 4) ? img_nv(KP, CM) % 4 : 0;
+}
+__host__ __device__ constexpr int img_floats(int KP, int CM)
+{
+    return img_tail(KP, CM) ? (img_nv(KP, CM) / 4) * 256 + img_tail(KP, CM) * 64
+                            : ((img_nv(KP, CM) + 3) & ~3) * 64;
+}
 
 __host__ __device__ inline int tile_sites(int K, int CM) { return K == 4 ? kTile4 : sites20(CM); }
 

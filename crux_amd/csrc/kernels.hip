@@ -203,9 +203,11 @@ __global__ void __launch_bounds__(256) k_pmat(const PJob *__restrict__ jobs, int
         // P_c[4 I + i][g * cols + a] for v = I * groups + g and
         // l = c * (64 / cm) + 4 a + i.
         const int G = 64 / cm, W = 16 / cm, NG = (kp + W - 1) / W, NV = (kp / 4) * NG;
-        const int n = ((NV + 3) & ~3) * 64;
+        const int n = img_floats(kp, cm), RT = img_tail(kp, cm), nb = (NV / 4) * 256;
         for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-            const int v = (idx >> 8) * 4 + (idx & 3), l = (idx >> 2) & 63;
+            // A float4 block, or the tail's [lane][RT] dwords.
+            const int v = RT && idx >= nb ? 4 * (NV / 4) + (idx - nb) % RT : (idx >> 8) * 4 + (idx & 3);
+            const int l = RT && idx >= nb ? (idx - nb) / RT : (idx >> 2) & 63;
             float val = 0.0f;
             if (v < NV) {
                 const int I = v / NG, g = v - I * NG, c = l / G, a = (l % G) >> 2;
@@ -1692,7 +1694,12 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     // float4 per lane of a prefetched P image (the split precision's low
     // image is loaded where it is used, so that its registers stay within
     // two waves per SIMD).
-    constexpr int NQ = PREC == kPrecF16x3 ? NV16P / 4 : img_words(KP, CM, PREC) / 256;
+    // Dwords per lane of an FP32 image's compact last block (0: the last
+    // block is a padded float4 block like the others).
+    constexpr int RT = PREC == kPrecF32 ? img_tail(KP, CM) : 0;
+    constexpr int NQ = PREC == kPrecF16x3 ? NV16P / 4
+                       : PREC == kPrecF32 ? (img_nv(KP, CM) + 3) / 4
+                                          : img_words(KP, CM, PREC) / 256;
     constexpr int PQ = NQ > NI ? NQ : NI;           // float4 of a prefetched child
     constexpr int WPR = G / 4;                      // code dwords per staged tip row
     constexpr int RPI = 64 / WPR;                   // tip rows per staging load
@@ -1798,10 +1805,33 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
     // Loads: a P image (PQ float4, a wave apart), a tip row (NI consecutive
     // float4 of the lane's (code, component) row), a CLV (NI float4 1 KiB
     // apart) and its exponent.
+    // The image's compact last block: RT dwords of the lane at voff (an
+    // offset with kOut set reads nothing).
+    auto load_tail = [&](rsrc_t r, uint32_t voff) -> crux_f32x4 {
+        crux_f32x4 t = {0.f, 0.f, 0.f, 0.f};
+        constexpr int soff = (NQ - 1) * 1024;
+        if constexpr (RT == 1) {
+            t[0] = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+        } else if constexpr (RT == 2) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0);
+            t[0] = __uint_as_float(v[0]);
+            t[1] = __uint_as_float(v[1]);
+        } else if constexpr (RT == 3) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)voff, soff, 0);
+            t[0] = __uint_as_float(v[0]);
+            t[1] = __uint_as_float(v[1]);
+            t[2] = __uint_as_float(v[2]);
+        }
+        return t;
+    };
     auto load_img = [&](rsrc_t r, crux_f32x4 (&pr)[PQ]) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            pr[q] = bload4(r, lane16, q * 1024);
+        for (int q = 0; q < NQ; ++q) {
+            if (RT != 0 && q == NQ - 1)
+                pr[q] = load_tail(r, (uint32_t)lane * RT * 4);
+            else
+                pr[q] = bload4(r, lane16, q * 1024);
+        }
     };
     auto load_tip = [&](rsrc_t r, uint32_t code, crux_f32x4 (&pr)[PQ]) {
         const uint32_t voff = code * (CM * KP * 4) + vtip;
@@ -1951,8 +1981,14 @@ k_trav20(const Entry *__restrict__ entries, const int32_t *__restrict__ block_en
         const uint32_t voff_hi = voff | ((mtip | mhbm) & kOut);
         const int sh = tip ? 4 : 10;
 #pragma unroll
-        for (int q = 0; q < PQ; ++q)
-            pre[j][q] = bload4(r, q < NI ? voff : voff_hi, (uint32_t)q << sh);
+        for (int q = 0; q < PQ; ++q) {
+            // An image's compact last block (past a tip row's or CLV's
+            // float4s, where those read nothing).
+            if (RT != 0 && q == NQ - 1)
+                pre[j][q] = load_tail(r, ((uint32_t)lane * RT * 4) | ((mtip | mhbm) & kOut));
+            else
+                pre[j][q] = bload4(r, q < NI ? voff : voff_hi, (uint32_t)q << sh);
+        }
         // The exponent of a cached child (an empty range otherwise).
         uint32_t ve[3] = {rn.slot(j)[0], rn.slot(j)[1], hbm ? rn.slot(j)[2] : 0u};
         he[j] = bload1(mk(ve), vexl, 0);
