@@ -12,6 +12,10 @@ lnL bit for bit):
   wait for their stores once per chunk;
 * 20/22 states: waves per SIMD the kernel's registers are held to (1 or 2).
 
+Where the context's split setting runs the class's plans over two launches
+(subtree blocks, then the updates above them), the tuner also times the
+best shape in one launch and keeps that for the class if it is faster.
+
 Which shape wins depends on the tree, the state and component counts and the
 pattern count, so the tuner times full traversals of the caller's own
 likelihood at every shape (with the device's own event timer around the
@@ -65,10 +69,24 @@ def grid(lik):
 
 def set_class_shape(lik, shape):
     """Set (depth, chunk, waves) for lik's class on its context (None: clear,
-    the context's values apply)."""
+    the context's values apply, split setting included)."""
     ctx = lik._ctx
     for key, v in zip(KEYS, shape if shape is not None else (-1, -1, -1)):
         ctx.set_class_tuning(lik.dim, _ncomp(lik), key, v)
+    if shape is None:
+        set_class_split(lik, None)
+
+
+def set_class_split(lik, value):
+    """Split setting of lik's class (CRUX_TUNE_SPLIT; 0: one launch, None:
+    the context's)."""
+    lik._ctx.set_class_tuning(lik.dim, _ncomp(lik), nat.CRUX_TUNE_SPLIT, -1 if value is None else value)
+
+
+def class_split(lik):
+    """The class's split setting (None: the context's applies)."""
+    v = lik._ctx.get_class_tuning(lik.dim, _ncomp(lik), nat.CRUX_TUNE_SPLIT)
+    return None if v < 0 else v
 
 
 def class_shape(lik):
@@ -106,9 +124,11 @@ def tune(lik, root=None, shapes=None, reps=8, use_cache=True):
     """
     key = _shape(lik)
     if use_cache and key in _cache:
-        best, times = _cache[key]
+        best, times, split = _cache[key]
         set_class_shape(lik, best)
+        set_class_split(lik, split)
         return best, dict(times)
+    set_class_split(lik, None)
     times = {}
     seen = set()
     for sh in shapes if shapes is not None else grid(lik):
@@ -121,7 +141,18 @@ def tune(lik, root=None, shapes=None, reps=8, use_cache=True):
         times[sh] = time_traversal(lik, root, reps=reps)
     best = min(times, key=lambda s: (times[s], s))
     set_class_shape(lik, best)
-    _cache[key] = (best, dict(times))
+    # Split plans (two traversal launches for the one class): one launch at
+    # the best shape, kept if faster.
+    split = None
+    ctx = lik._ctx
+    lik.flush()
+    lik.lnL(root)
+    if ctx.last_launches()[0] > 1:
+        set_class_split(lik, 0)
+        if time_traversal(lik, root, reps=reps) < times[best]:
+            split = 0
+    set_class_split(lik, split)
+    _cache[key] = (best, dict(times), split)
     return best, times
 
 

@@ -81,7 +81,7 @@ struct crux_ctx {
     int split20 = 3;               // 20/22-state split: blocks per SIMD aimed at (0: never)
     // Per-class settings of the three shape keys (-1: the context's value),
     // indexed by class_index(K, CM).
-    int ctune[kNumClasses][3];
+    int ctune[kNumClasses][4];   // depth, chunk, waves, split (-1: the context's)
     float *d_zero = nullptr;       // kZeroBytes of zeros (20/22-state kernel)
     bool timing = false;
     double last_ms[4] = {0, 0, 0, 0};
@@ -1265,7 +1265,7 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
     // work per SIMD (enough for the balance the subtree blocks bring to
     // outweigh the second launch).
     int cls_split[kNumClasses] = {};
-    if (!plain && ctx->split20 > 0) {
+    if (!plain) {
         double wu[kNumClasses] = {};
         int cls_k[kNumClasses] = {};
         for (Item &it : items) {
@@ -1284,8 +1284,12 @@ static int submit_to(crux_ctx *ctx, Staging &S, bool fixed, std::vector<crux_lik
         // aimed at: 154k at 256 CUs, where a 128-taxon plan over 947 tiles
         // has 119k and runs 12% slower split).
         for (int ci = 0; ci < kNumClasses; ci++) {
+            // The class's own setting, else the context's.
+            const int sv = ctx->ctune[ci][3] >= 0 ? ctx->ctune[ci][3] : ctx->split20;
+            if (sv <= 0)
+                continue;
             const bool k4 = cls_k[ci] == 4;
-            const double want = (double)ctx->split20 * (k4 ? 0.5 * ctx->ncu : 4.0 * ctx->ncu);
+            const double want = (double)sv * (k4 ? 0.5 * ctx->ncu : 4.0 * ctx->ncu);
             const double tl = (double)cls_tiles[ci];
             if (wu[ci] > 0 && (tl < want || (k4 ? wu[ci] >= 400 * want
                                                  : tl < 2 * want && wu[ci] >= 200 * want)))
@@ -2061,6 +2065,10 @@ static int check_shape_value(int key, int value)
         if (value < 0 || value > 2)
             return set_error(CRUX_EINVAL, "waves per SIMD %d (0 default, 1 or 2)", value);
         return CRUX_OK;
+    case CRUX_TUNE_SPLIT:
+        if (value < 0 || value > 16)
+            return set_error(CRUX_EINVAL, "split %d (0 to 16)", value);
+        return CRUX_OK;
     default:
         return set_error(CRUX_EINVAL, "unknown tuning key %d", key);
     }
@@ -2130,7 +2138,11 @@ static int class_slot(int K, int C, int key, int *ci, int *ki)
     *ci = cm > 0 ? class_index(K, cm) : -1;
     if (*ci < 0)
         return set_error(CRUX_EINVAL, "no traversal class for K=%d C=%d", K, C);
-    *ki = key == CRUX_TUNE_STACK_DEPTH ? 0 : key == CRUX_TUNE_CHUNK ? 1 : key == CRUX_TUNE_WAVES ? 2 : -1;
+    *ki = key == CRUX_TUNE_STACK_DEPTH ? 0
+          : key == CRUX_TUNE_CHUNK     ? 1
+          : key == CRUX_TUNE_WAVES     ? 2
+          : key == CRUX_TUNE_SPLIT     ? 3
+                                       : -1;
     if (*ki < 0)
         return set_error(CRUX_EINVAL, "tuning key %d is not a launch-shape key", key);
     return CRUX_OK;

@@ -287,7 +287,8 @@ int crux_ctx_set_mode(crux_ctx *ctx, int mode);
  *   aimed at) updates and half the plan, one block per subtree and tile in
  *   a first launch, and the updates above them in a second launch that
  *   reads the subtrees' results from their slots.  Same results, bit for
- *   bit, either way.
+ *   bit, either way.  Also a class key (crux_ctx_set_class_tuning), which
+ *   crux_amd.autotune sets to 0 where one launch measures faster.
  * Settings apply to batches submitted after the call; do not change them
  * while another thread submits on the context. */
 enum { CRUX_PREC_F32 = 0, CRUX_PREC_BF16 = 1, CRUX_PREC_F16 = 2, CRUX_PREC_F16X3 = 3 };
@@ -302,10 +303,11 @@ enum {
 int crux_ctx_set_tuning(crux_ctx *ctx, int key, int value);
 int crux_ctx_get_tuning(const crux_ctx *ctx, int key, int *value);
 
-/* The same shape keys (not precision) for one traversal class: K states and
- * C components (the class of the smallest instantiated component count >=
- * C).  value -1 clears the class's setting (the context's value applies).
- * A batch mixing classes launches each with its own shape. */
+/* The same shape keys and CRUX_TUNE_SPLIT (not precision or short plans)
+ * for one traversal class: K states and C components (the class of the
+ * smallest instantiated component count >= C).  value -1 clears the class's
+ * setting (the context's value applies).  A batch mixing classes launches
+ * each with its own shape. */
 int crux_ctx_set_class_tuning(crux_ctx *ctx, int K, int C, int key, int value);
 int crux_ctx_get_class_tuning(const crux_ctx *ctx, int K, int C, int key, int *value);
 

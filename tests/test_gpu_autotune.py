@@ -116,3 +116,32 @@ def test_tuner_keeps_a_timed_shape_and_caches(kind):
     finally:
         autotune.set_class_shape(lik, None)
         autotune.clear()
+
+
+@pytest.mark.parametrize("kind", ["dna", "aa"])
+def test_tuner_decides_the_split(kind):
+    """A class whose plans the context splits over two launches: the tuner
+    keeps the split or one launch for the class, whichever its timings
+    favour, the lnL is the same bit for bit either way, and clearing the
+    class's shape clears its split setting."""
+    print("autotune split", kind, flush=True)
+    prob = synth.make_problem(120, 1500 if kind == "aa" else 3000, 31, kind=kind, ncat=4)
+    lik = Lik(prob.tree, prob.alignment, ncat=4)
+    synth.configure(lik, prob.model)
+    ctx = lik._ctx
+    try:
+        autotune.clear()
+        lik.flush()
+        ref = lik.lnL()
+        assert ctx.last_launches()[0] == 2
+        autotune.tune(lik, reps=2)
+        split = autotune.class_split(lik)
+        assert split in (None, 0), split
+        lik.flush()
+        assert lik.lnL() == ref
+        assert ctx.last_launches()[0] == (2 if split is None else 1)
+        autotune.set_class_shape(lik, None)
+        assert autotune.class_split(lik) is None
+    finally:
+        autotune.set_class_shape(lik, None)
+        autotune.clear()
